@@ -1,0 +1,268 @@
+"""ctypes binding for the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker, never as the measured or
+shipped path.  Complex arrays are numpy complex64 (interleaved float pairs,
+the layout of std::complex<float>).
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build() -> None:
+    """Compile liboracle.so (gcc, the reference's flags plus -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ct.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+P = ct.c_void_p
+I = ct.c_int
+D = ct.c_double
+
+
+def _declare(L):
+    L.orc_random_int.restype = I
+    L.orc_random_int.argtypes = [ct.POINTER(ct.c_uint64)]
+    L.orc_attic_uvw.argtypes = [I, I, ct.POINTER(ct.c_uint64), P, P, P]
+    L.orc_attic_freq.argtypes = [I, P]
+    L.orc_attic_initC.argtypes = [D, D, I, I, I, ct.POINTER(I), ct.POINTER(D), P]
+    L.orc_attic_initCOffset.argtypes = [I, P, P, P, I, P, D, D, I, I, I, I, P, P, P]
+    L.orc_current_initC.restype = ct.c_long
+    L.orc_current_initC.argtypes = [D, I, I, I, D, P, P, P]
+    L.orc_current_initCOffset.argtypes = [I, P, P, P, I, P, D, D, I, I, I, P, P, P, P, P, P, P]
+    L.orc_current_params.restype = I
+    L.orc_current_params.argtypes = [I, ct.POINTER(I), ct.POINTER(D), ct.POINTER(I),
+                                      ct.POINTER(I), ct.POINTER(I), ct.POINTER(D),
+                                      ct.POINTER(ct.c_float)]
+    L.orc_current_uvw.restype = I
+    L.orc_current_uvw.argtypes = [ct.c_float, ct.POINTER(ct.c_uint64), P, P, P]
+    L.orc_grid.argtypes = [I, P, P, P, P, P, P, I, P, P, I]
+    L.orc_degrid.argtypes = [I, P, I, P, P, P, P, P, I, P, P]
+    L.orc_grid_f64.argtypes = [I, P, P, P, P, P, P, I, P, P, I]
+    L.orc_degrid_f64.argtypes = [I, P, I, P, P, P, P, P, I, P, P]
+    L.orc_grid_points.restype = ct.c_longlong
+    L.orc_grid_points.argtypes = [I, P, P, I]
+    L.orc_cpu_baseline.restype = I
+    L.orc_cpu_baseline.argtypes = [I, I, I, P, P, P, P, P, P, I, P, I,
+                                   ct.POINTER(D), ct.POINTER(D)]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(P)
+
+
+def _c(a, dtype):
+    if a is None:
+        return None
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# ---- set-up -----------------------------------------------------------------
+
+class LCG:
+    """Benchmark::randomInt state (seed next = 1)."""
+
+    def __init__(self, seed: int = 1):
+        self.state = ct.c_uint64(seed)
+
+    def next(self) -> int:
+        return lib().orc_random_int(ct.byref(self.state))
+
+
+def attic_uvw(n, baseline=2000, lcg: LCG | None = None):
+    lcg = lcg or LCG()
+    u = np.empty(n, np.float64)
+    v = np.empty(n, np.float64)
+    w = np.empty(n, np.float64)
+    lib().orc_attic_uvw(n, baseline, ct.byref(lcg.state), _p(u), _p(v), _p(w))
+    return u, v, w
+
+
+def attic_freq(nChan):
+    f = np.empty(nChan, np.float64)
+    lib().orc_attic_freq(nChan, _p(f))
+    return f
+
+
+def attic_initC(freq0, cellSize, wSize, baseline, overSample=8, table=True):
+    sup = I()
+    wcs = D()
+    lib().orc_attic_initC(freq0, cellSize, wSize, baseline, overSample,
+                          ct.byref(sup), ct.byref(wcs), None)
+    C = None
+    if table:
+        s = 2 * sup.value + 1
+        C = np.empty(s * s * overSample * overSample * wSize, np.complex64)
+        lib().orc_attic_initC(freq0, cellSize, wSize, baseline, overSample,
+                              ct.byref(sup), ct.byref(wcs), _p(C))
+    return sup.value, wcs.value, C
+
+
+def attic_initCOffset(u, v, w, freq, cellSize, wCellSize, wSize, gSize, support,
+                      overSample=8):
+    u, v, w, freq = (_c(x, np.float64) for x in (u, v, w, freq))
+    n = u.size * freq.size
+    iu = np.empty(n, np.int32)
+    iv = np.empty(n, np.int32)
+    co = np.empty(n, np.int32)
+    lib().orc_attic_initCOffset(u.size, _p(u), _p(v), _p(w), freq.size, _p(freq),
+                                cellSize, wCellSize, wSize, gSize, support,
+                                overSample, _p(iu), _p(iv), _p(co))
+    return iu, iv, co
+
+
+def current_params(runType):
+    g, ws, sup, os_ = I(), I(), I(), I()
+    uvc, wcs = D(), D()
+    bl = ct.c_float()
+    rc = lib().orc_current_params(runType, ct.byref(g), ct.byref(uvc), ct.byref(ws),
+                                  ct.byref(sup), ct.byref(os_), ct.byref(wcs),
+                                  ct.byref(bl))
+    if rc:
+        raise ValueError(f"unsupported runType {runType}")
+    return dict(gSize=g.value, uvCellSize=uvc.value, wSize=ws.value,
+                support=sup.value, overSample=os_.value, wCellSize=wcs.value,
+                baseline=bl.value)
+
+
+def current_uvw(baseline, lcg: LCG | None = None):
+    lcg = lcg or LCG()
+    n = lib().orc_current_uvw(baseline, ct.byref(lcg.state), None, None, None)
+    u = np.empty(n, np.float64)
+    v = np.empty(n, np.float64)
+    w = np.empty(n, np.float64)
+    lib().orc_current_uvw(baseline, ct.byref(lcg.state), _p(u), _p(v), _p(w))
+    return u, v, w
+
+
+def current_initC(uvCellSize, wSize, support, overSample, wCellSize, table=True):
+    sSize = np.zeros(wSize, np.int32)
+    cOffset0 = np.zeros(wSize, np.int32)
+    n = lib().orc_current_initC(uvCellSize, wSize, support, overSample, wCellSize,
+                                _p(sSize), _p(cOffset0), None)
+    C = None
+    if table:
+        C = np.empty(n, np.complex64)
+        lib().orc_current_initC(uvCellSize, wSize, support, overSample, wCellSize,
+                                _p(sSize), _p(cOffset0), _p(C))
+    return sSize, cOffset0, C
+
+
+def current_initCOffset(u, v, w, wavenumber, uvCellSize, wCellSize, wSize, gSize,
+                        overSample, sSize, cOffset0):
+    u, v, w, wavenumber = (_c(x, np.float64) for x in (u, v, w, wavenumber))
+    sSize = _c(sSize, np.int32)
+    cOffset0 = _c(cOffset0, np.int32)
+    n = u.size * wavenumber.size
+    iu = np.empty(n, np.int32)
+    iv = np.empty(n, np.int32)
+    wp = np.empty(n, np.int32)
+    co = np.empty(n, np.int32)
+    npp = np.zeros(wSize, np.int64)
+    lib().orc_current_initCOffset(u.size, _p(u), _p(v), _p(w), wavenumber.size,
+                                  _p(wavenumber), uvCellSize, wCellSize, wSize, gSize,
+                                  overSample, _p(sSize), _p(cOffset0), _p(iu), _p(iv),
+                                  _p(wp), _p(co), _p(npp))
+    return iu, iv, wp, co, npp
+
+
+# ---- kernels ----------------------------------------------------------------
+
+def _vis_args(iu, iv, cOffset, wPlane, sSize):
+    return (_c(iu, np.int32), _c(iv, np.int32), _c(cOffset, np.int32),
+            _c(wPlane, np.int32), _c(sSize, np.int32))
+
+
+def grid(data, iu, iv, cOffset, C, gSize, sSize, wPlane=None, out=None):
+    """gridKernel: accumulates into `out` (complex64 gSize x gSize, created zero)."""
+    data = _c(data, np.complex64)
+    C = _c(C, np.complex64)
+    iu, iv, cOffset, wPlane, sSz = _vis_args(iu, iv, cOffset, wPlane,
+                                             sSize if wPlane is not None else None)
+    sUni = 0 if wPlane is not None else int(sSize)
+    if out is None:
+        out = np.zeros((gSize, gSize), np.complex64)
+    lib().orc_grid(data.size, _p(data), _p(iu), _p(iv), _p(cOffset), _p(wPlane),
+                   _p(sSz), sUni, _p(C), _p(out), gSize)
+    return out
+
+
+def degrid(grid_, iu, iv, cOffset, C, sSize, wPlane=None):
+    grid_ = _c(grid_, np.complex64)
+    C = _c(C, np.complex64)
+    gSize = grid_.shape[0]
+    iu, iv, cOffset, wPlane, sSz = _vis_args(iu, iv, cOffset, wPlane,
+                                             sSize if wPlane is not None else None)
+    sUni = 0 if wPlane is not None else int(sSize)
+    out = np.empty(iu.size, np.complex64)
+    lib().orc_degrid(iu.size, _p(grid_), gSize, _p(iu), _p(iv), _p(cOffset),
+                     _p(wPlane), _p(sSz), sUni, _p(C), _p(out))
+    return out
+
+
+def grid_f64(data, iu, iv, cOffset, C, gSize, sSize, wPlane=None):
+    data = _c(data, np.complex128)
+    C = _c(C, np.complex64)
+    iu, iv, cOffset, wPlane, sSz = _vis_args(iu, iv, cOffset, wPlane,
+                                             sSize if wPlane is not None else None)
+    sUni = 0 if wPlane is not None else int(sSize)
+    out = np.zeros((gSize, gSize), np.complex128)
+    lib().orc_grid_f64(data.size, _p(data), _p(iu), _p(iv), _p(cOffset), _p(wPlane),
+                       _p(sSz), sUni, _p(C), _p(out), gSize)
+    return out
+
+
+def degrid_f64(grid_, iu, iv, cOffset, C, sSize, wPlane=None):
+    grid_ = _c(grid_, np.complex128)
+    C = _c(C, np.complex64)
+    gSize = grid_.shape[0]
+    iu, iv, cOffset, wPlane, sSz = _vis_args(iu, iv, cOffset, wPlane,
+                                             sSize if wPlane is not None else None)
+    sUni = 0 if wPlane is not None else int(sSize)
+    out = np.empty(iu.size, np.complex128)
+    lib().orc_degrid_f64(iu.size, _p(grid_), gSize, _p(iu), _p(iv), _p(cOffset),
+                         _p(wPlane), _p(sSz), sUni, _p(C), _p(out))
+    return out
+
+
+def grid_points(n, sSize, wPlane=None):
+    if wPlane is None:
+        return int(n) * int(sSize) * int(sSize)
+    wPlane = _c(wPlane, np.int32)
+    sSize = _c(sSize, np.int32)
+    return int(lib().orc_grid_points(wPlane.size, _p(wPlane), _p(sSize), 0))
+
+
+def cpu_baseline(nThreads, nVis, data, iu, iv, cOffset, C, gSize, sSize, wPlane=None):
+    """tConvolveMPI-style replicated CPU run; returns (t_grid_s, t_degrid_s)."""
+    data = _c(data, np.complex64)
+    C = _c(C, np.complex64)
+    iu, iv, cOffset, wPlane, sSz = _vis_args(iu, iv, cOffset, wPlane,
+                                             sSize if wPlane is not None else None)
+    sUni = 0 if wPlane is not None else int(sSize)
+    tg, td = D(), D()
+    rc = lib().orc_cpu_baseline(nThreads, nVis, iu.size, _p(data), _p(iu), _p(iv),
+                                _p(cOffset), _p(wPlane), _p(sSz), sUni, _p(C), gSize,
+                                ct.byref(tg), ct.byref(td))
+    if rc:
+        raise MemoryError(f"cpu baseline failed (rc={rc})")
+    return tg.value, td.value
