@@ -1,0 +1,544 @@
+// Host side of libtcv: context, memory, timing, the Benchmark set-up
+// generators, and the C ABI entry points declared in include/tcv.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "tcv_internal.h"
+
+tcv_status tcv_fail(tcv_ctx *ctx, tcv_status s, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return s;
+}
+
+ProfScope::ProfScope(tcv_ctx *c, const char *fam) : ctx(c), family(fam)
+{
+    if (!ctx->profiling) return;
+    auto take = [&]() {
+        hipEvent_t e = nullptr;
+        if (!ctx->pool.empty()) {
+            e = ctx->pool.back();
+            ctx->pool.pop_back();
+        } else if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+        }
+        return e;
+    };
+    a = take();
+    b = take();
+    if (a) (void)hipEventRecord(a, ctx->stream);
+}
+
+ProfScope::~ProfScope()
+{
+    if (!ctx->profiling || !a || !b) return;
+    (void)hipEventRecord(b, ctx->stream);
+    ctx->pending.push_back({a, b, family});
+}
+
+static tcv_status drain_profile(tcv_ctx *ctx)
+{
+    for (auto &p : ctx->pending) {
+        TCV_HIP(ctx, hipEventSynchronize(p.b));
+        float ms = 0.f;
+        TCV_HIP(ctx, hipEventElapsedTime(&ms, p.a, p.b));
+        auto &st = ctx->stats[p.family];
+        st.launches += 1;
+        st.ms += ms;
+        ctx->pool.push_back(p.a);
+        ctx->pool.push_back(p.b);
+    }
+    ctx->pending.clear();
+    return TCV_OK;
+}
+
+extern "C" {
+
+const char *tcv_status_string(tcv_status s)
+{
+    switch (s) {
+    case TCV_OK: return "ok";
+    case TCV_ERR_INVALID: return "invalid argument";
+    case TCV_ERR_RANGE: return "index out of range";
+    case TCV_ERR_DEVICE: return "device error";
+    case TCV_ERR_NOMEM: return "out of device memory";
+    case TCV_ERR_BUDGET: return "HBM budget exceeded";
+    case TCV_ERR_UNSUPPORTED: return "unsupported";
+    case TCV_ERR_CANCELLED: return "cancelled";
+    }
+    return "unknown status";
+}
+
+tcv_status tcv_create(int device, tcv_ctx **out)
+{
+    if (!out) return TCV_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return TCV_ERR_DEVICE;
+    if (device < 0 || device >= n) return TCV_ERR_INVALID;
+    tcv_ctx *ctx = new tcv_ctx();
+    ctx->device = device;
+    auto fail = [&](tcv_status s) { delete ctx; return s; };
+    if (hipSetDevice(device) != hipSuccess) return fail(TCV_ERR_DEVICE);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(TCV_ERR_DEVICE);
+    ctx->cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(TCV_ERR_DEVICE);
+    if (hipEventCreate(&ctx->timer0) != hipSuccess || hipEventCreate(&ctx->timer1) != hipSuccess)
+        return fail(TCV_ERR_DEVICE);
+    if (hipMalloc(&ctx->d_counters, 8 * sizeof(unsigned long long)) != hipSuccess)
+        return fail(TCV_ERR_NOMEM);
+    if (hipMemset(ctx->d_counters, 0, 8 * sizeof(unsigned long long)) != hipSuccess)
+        return fail(TCV_ERR_DEVICE);
+    *out = ctx;
+    return TCV_OK;
+}
+
+void tcv_destroy(tcv_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &p : ctx->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : ctx->pool) (void)hipEventDestroy(e);
+    if (ctx->timer0) (void)hipEventDestroy(ctx->timer0);
+    if (ctx->timer1) (void)hipEventDestroy(ctx->timer1);
+    if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *tcv_last_error(const tcv_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int tcv_device_cus(const tcv_ctx *ctx) { return ctx ? ctx->cus : 0; }
+
+tcv_status tcv_synchronize(tcv_ctx *ctx)
+{
+    if (!ctx) return TCV_ERR_INVALID;
+    TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return TCV_OK;
+}
+
+tcv_status tcv_malloc(tcv_ctx *ctx, size_t bytes, void **d_ptr)
+{
+    if (!ctx || !d_ptr) return TCV_ERR_INVALID;
+    *d_ptr = nullptr;
+    TCV_HIP(ctx, hipSetDevice(ctx->device));
+    hipError_t e = hipMalloc(d_ptr, std::max<size_t>(bytes, 256));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return tcv_fail(ctx, TCV_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    return TCV_OK;
+}
+
+tcv_status tcv_free(tcv_ctx *ctx, void *d_ptr)
+{
+    if (!ctx) return TCV_ERR_INVALID;
+    if (!d_ptr) return TCV_OK;
+    TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    TCV_HIP(ctx, hipFree(d_ptr));
+    return TCV_OK;
+}
+
+tcv_status tcv_upload(tcv_ctx *ctx, void *d_dst, const void *src, size_t bytes)
+{
+    if (!ctx || (bytes && (!d_dst || !src))) return TCV_ERR_INVALID;
+    if (!bytes) return TCV_OK;
+    TCV_HIP(ctx, hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return TCV_OK;
+}
+
+tcv_status tcv_download(tcv_ctx *ctx, void *dst, const void *d_src, size_t bytes)
+{
+    if (!ctx || (bytes && (!dst || !d_src))) return TCV_ERR_INVALID;
+    if (!bytes) return TCV_OK;
+    TCV_HIP(ctx, hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return TCV_OK;
+}
+
+tcv_status tcv_zero(tcv_ctx *ctx, void *d_ptr, size_t bytes)
+{
+    if (!ctx || (bytes && !d_ptr) || (bytes % 4)) return TCV_ERR_INVALID;
+    return launch_fill_u32(ctx, d_ptr, 0u, bytes / 4);
+}
+
+tcv_status tcv_timer_start(tcv_ctx *ctx)
+{
+    if (!ctx) return TCV_ERR_INVALID;
+    TCV_HIP(ctx, hipEventRecord(ctx->timer0, ctx->stream));
+    return TCV_OK;
+}
+
+tcv_status tcv_timer_stop(tcv_ctx *ctx, double *ms)
+{
+    if (!ctx || !ms) return TCV_ERR_INVALID;
+    TCV_HIP(ctx, hipEventRecord(ctx->timer1, ctx->stream));
+    TCV_HIP(ctx, hipEventSynchronize(ctx->timer1));
+    float f = 0.f;
+    TCV_HIP(ctx, hipEventElapsedTime(&f, ctx->timer0, ctx->timer1));
+    *ms = f;
+    return TCV_OK;
+}
+
+tcv_status tcv_set_profiling(tcv_ctx *ctx, int on)
+{
+    if (!ctx) return TCV_ERR_INVALID;
+    ctx->profiling = on != 0;
+    return TCV_OK;
+}
+
+tcv_status tcv_kernel_stats(tcv_ctx *ctx, const char *family, long *launches, double *total_ms)
+{
+    if (!ctx || !family) return TCV_ERR_INVALID;
+    tcv_status st = drain_profile(ctx);
+    if (st != TCV_OK) return st;
+    auto it = ctx->stats.find(family);
+    if (launches) *launches = it == ctx->stats.end() ? 0 : it->second.launches;
+    if (total_ms) *total_ms = it == ctx->stats.end() ? 0.0 : it->second.ms;
+    return TCV_OK;
+}
+
+tcv_status tcv_reset_stats(tcv_ctx *ctx)
+{
+    if (!ctx) return TCV_ERR_INVALID;
+    tcv_status st = drain_profile(ctx);
+    ctx->stats.clear();
+    return st;
+}
+
+// ---- a6 ----------------------------------------------------------------------
+
+tcv_status tcv_init_offsets(tcv_ctx *ctx, const double *d_u, const double *d_v, const double *d_w,
+                            int nSamples, const double *freq, int nChan, double cellSize,
+                            double wCellSize, int wSize, int gSize, int overSample,
+                            const int *sSize, const int *cOffset0, int *d_iu, int *d_iv,
+                            int *d_wPlane, int *d_cOffset, long long *numPerPlane,
+                            long long *nOutOfRange)
+{
+    if (!ctx || nSamples < 0 || nChan < 1 || !freq || wSize < 1 || gSize < 1 || !sSize ||
+        overSample < 1 || cellSize <= 0 || (nSamples > 0 && (!d_u || !d_v || !d_w || !d_iu || !d_iv || !d_cOffset)))
+        return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_init_offsets: bad argument");
+    if ((long long)nSamples * nChan > 0x7fffffffLL)
+        return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "more than 2^31-1 visibilities");
+    if (cOffset0 == nullptr && wCellSize <= 0)
+        return tcv_fail(ctx, TCV_ERR_INVALID, "attic layout needs wCellSize > 0");
+    TCV_HIP(ctx, hipSetDevice(ctx->device));
+    const int nS = cOffset0 ? wSize : 1;
+    // small staging block: freq | sSize | cOffset0 | numPerPlane | bad
+    const size_t fB = sizeof(double) * nChan, sB = sizeof(int) * nS, oB = sizeof(int) * wSize;
+    const size_t nB = sizeof(unsigned long long) * wSize;
+    const size_t total = fB + sB + oB + nB + 64;
+    char *d = nullptr;
+    TCV_HIP(ctx, hipMalloc((void **)&d, total + 64));
+    double *d_freq = (double *)d;
+    int *d_s = (int *)(d + fB);
+    int *d_o = (int *)(d + fB + sB);
+    size_t nOff = (fB + sB + oB + 7) & ~size_t(7);
+    unsigned long long *d_npp = (unsigned long long *)(d + nOff);
+    unsigned long long *d_bad = d_npp + wSize;
+    auto cleanup = [&]() { (void)hipFree(d); };
+    hipError_t e = hipMemcpyAsync(d_freq, freq, fB, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_s, sSize, sB, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && cOffset0)
+        e = hipMemcpyAsync(d_o, cOffset0, oB, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+        cleanup();
+        return tcv_fail(ctx, TCV_ERR_DEVICE, hipGetErrorString(e));
+    }
+    tcv_status st = launch_fill_u32(ctx, d_npp, 0, (nB + 8) / 4);
+    if (st == TCV_OK)
+        st = launch_offsets(ctx, d_u, d_v, d_w, nSamples, d_freq, nChan, cellSize, wCellSize,
+                            wSize, gSize, overSample, d_s, cOffset0 ? d_o : nullptr, d_iu, d_iv,
+                            d_wPlane, d_cOffset, d_npp, d_bad);
+    std::vector<unsigned long long> npp(wSize + 1);
+    if (st == TCV_OK) {
+        e = hipMemcpyAsync(npp.data(), d_npp, nB + 8, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st = tcv_fail(ctx, TCV_ERR_DEVICE, hipGetErrorString(e));
+    }
+    cleanup();
+    if (st != TCV_OK) return st;
+    if (numPerPlane)
+        for (int k = 0; k < wSize; k++) numPerPlane[k] = (long long)npp[k];
+    const long long bad = (long long)npp[wSize];
+    if (nOutOfRange) *nOutOfRange = bad;
+    if (bad) return tcv_fail(ctx, TCV_ERR_RANGE, std::to_string(bad) + " visibilities out of range");
+    return TCV_OK;
+}
+
+}  // extern "C"
+
+// ---- geometry helpers ------------------------------------------------------------
+
+static tcv_status check_geometry(tcv_ctx *ctx, const tcv_geometry *g, int *maxS)
+{
+    if (!ctx) return TCV_ERR_INVALID;
+    if (!g || g->nVis < 0 || g->gSize < 1 || g->nPlanes < 1 || !g->sSize || g->cSize < 0)
+        return tcv_fail(ctx, TCV_ERR_INVALID, "bad geometry");
+    if (g->nVis > 0 && (!g->d_iu || !g->d_iv || !g->d_cOffset))
+        return tcv_fail(ctx, TCV_ERR_INVALID, "geometry index arrays missing");
+    if (!g->d_wPlane && g->nPlanes != 1)
+        return tcv_fail(ctx, TCV_ERR_INVALID, "uniform geometry needs nPlanes == 1");
+    int m = 0;
+    for (int k = 0; k < g->nPlanes; k++) {
+        if (g->sSize[k] < 1) return tcv_fail(ctx, TCV_ERR_INVALID, "kernel width < 1");
+        m = std::max(m, g->sSize[k]);
+    }
+    *maxS = m;
+    return TCV_OK;
+}
+
+// Uploads the per-plane widths into a small device buffer when needed.
+struct GeomUpload {
+    int *d_s = nullptr;
+    ~GeomUpload() { if (d_s) (void)hipFree(d_s); }
+};
+
+static tcv_status make_geom_dev(tcv_ctx *ctx, const tcv_geometry *g, GeomUpload &up, GeomDev &gd)
+{
+    gd.nVis = g->nVis;
+    gd.iu = g->d_iu;
+    gd.iv = g->d_iv;
+    gd.cOffset = g->d_cOffset;
+    gd.wPlane = g->d_wPlane;
+    gd.nPlanes = g->nPlanes;
+    gd.sUniform = g->sSize[0];
+    gd.gSize = g->gSize;
+    gd.cSize = g->cSize;
+    gd.sSize = nullptr;
+    if (g->d_wPlane) {
+        TCV_HIP(ctx, hipMalloc((void **)&up.d_s, sizeof(int) * g->nPlanes));
+        TCV_HIP(ctx, hipMemcpyAsync(up.d_s, g->sSize, sizeof(int) * g->nPlanes,
+                                    hipMemcpyHostToDevice, ctx->stream));
+        gd.sSize = up.d_s;
+    }
+    return TCV_OK;
+}
+
+extern "C" {
+
+tcv_status tcv_grid_plain(tcv_ctx *ctx, const tcv_geometry *g, const float *d_data,
+                          const float *d_C, float *d_grid)
+{
+    int maxS = 0;
+    tcv_status st = check_geometry(ctx, g, &maxS);
+    if (st != TCV_OK) return st;
+    if (g->nVis > 0 && (!d_data || !d_C || !d_grid))
+        return tcv_fail(ctx, TCV_ERR_INVALID, "null data, C or grid");
+    GeomUpload up;
+    GeomDev gd;
+    if ((st = make_geom_dev(ctx, g, up, gd)) != TCV_OK) return st;
+    st = launch_grid_plain(ctx, gd, maxS, (const float2 *)d_data, (const float2 *)d_C,
+                           (float2 *)d_grid);
+    if (up.d_s) TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return st;
+}
+
+tcv_status tcv_degrid_plain(tcv_ctx *ctx, const tcv_geometry *g, const float *d_grid,
+                            const float *d_C, float *d_data)
+{
+    int maxS = 0;
+    tcv_status st = check_geometry(ctx, g, &maxS);
+    if (st != TCV_OK) return st;
+    if (g->nVis > 0 && (!d_data || !d_C || !d_grid))
+        return tcv_fail(ctx, TCV_ERR_INVALID, "null data, C or grid");
+    GeomUpload up;
+    GeomDev gd;
+    if ((st = make_geom_dev(ctx, g, up, gd)) != TCV_OK) return st;
+    st = launch_degrid_plain(ctx, gd, (const float2 *)d_grid, (const float2 *)d_C,
+                             (float2 *)d_data);
+    if (up.d_s) TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return st;
+}
+
+tcv_status tcv_range_errors(tcv_ctx *ctx, long long *count)
+{
+    if (!ctx || !count) return TCV_ERR_INVALID;
+    unsigned long long c = 0;
+    TCV_HIP(ctx, hipMemcpyAsync(&c, ctx->d_counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+    TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *count = (long long)c;
+    return launch_fill_u32(ctx, ctx->d_counters, 0, 2);
+}
+
+// ---- plans -------------------------------------------------------------------
+
+static void auto_tuning(int maxS, int gSize, int nVis, tcv_tuning *t)
+{
+    if (t->tile_w <= 0) t->tile_w = maxS >= 48 ? 64 : maxS >= 24 ? 32 : maxS >= 12 ? 16 : 8;
+    if (t->tile_h <= 0) t->tile_h = 64;
+    if (t->chunk <= 0) t->chunk = 2048;
+    if (t->slice_keys < 0) t->slice_keys = 0;
+    if (t->phases <= 0) t->phases = 1;
+    (void)gSize;
+    (void)nVis;
+}
+
+static void free_plan(tcv_plan *p)
+{
+    for (void *q : {(void *)p->d_sSize, (void *)p->entries, (void *)p->items, (void *)p->counts,
+                    (void *)p->starts, (void *)p->cursor, (void *)p->itemCount,
+                    (void *)p->itemStart, (void *)p->meta, (void *)p->scanTmp})
+        if (q) (void)hipFree(q);
+}
+
+tcv_status tcv_plan_create(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tuning,
+                           tcv_plan **out)
+{
+    if (!out) return TCV_ERR_INVALID;
+    *out = nullptr;
+    int maxS = 0;
+    tcv_status st = check_geometry(ctx, g, &maxS);
+    if (st != TCV_OK) return st;
+    if (g->gSize > 65535) return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tiled path needs gSize <= 65535");
+    if (g->cSize * 8 >= 0x7fffffffLL)
+        return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tiled path needs a C table below 2 GiB");
+    tcv_tuning t = tuning ? *tuning : tcv_tuning{0, 0, 0, 0, 0};
+    auto_tuning(maxS, g->gSize, g->nVis, &t);
+    if (!tiled_config_supported(t.tile_w, t.tile_h))
+        return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");
+    if (t.chunk < 1 || t.phases < 1 || t.slice_keys < 0)
+        return tcv_fail(ctx, TCV_ERR_INVALID, "bad tuning");
+    TCV_HIP(ctx, hipSetDevice(ctx->device));
+
+    tcv_plan *p = new tcv_plan();
+    PlanShape &sh = p->sh;
+    sh.tw = t.tile_w;
+    sh.th = t.tile_h;
+    sh.chunk = t.chunk;
+    sh.k2 = std::max(1, t.slice_keys);
+    sh.phases = std::min(t.phases, sh.k2);
+    sh.nTX = (g->gSize + sh.tw - 1) / sh.tw;
+    sh.nTY = (g->gSize + sh.th - 1) / sh.th;
+    sh.nTiles = sh.nTX * sh.nTY;
+    sh.maxS = maxS;
+    const long long perVis = (long long)((maxS + sh.tw - 2) / sh.tw + 1) * ((maxS + sh.th - 2) / sh.th + 1);
+    sh.maxEntries = std::max<long long>(1, (long long)g->nVis * perVis);
+    sh.nKeys = (long long)sh.nTiles * sh.k2;
+    if (sh.maxEntries >= 0x7fffffffLL || sh.nKeys >= 0x7fffffffLL) {
+        delete p;
+        return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "plan too large for 32-bit entry indices");
+    }
+    p->maxItems = sh.maxEntries / sh.chunk + (long long)sh.phases * sh.nTiles + 1;
+    p->nVis = g->nVis;
+    p->gSize = g->gSize;
+    p->nPlanes = g->nPlanes;
+    p->cSize = g->cSize;
+    p->sSizeHost.assign(g->sSize, g->sSize + g->nPlanes);
+    const long long slots = (long long)sh.phases * sh.nTiles;
+    p->scanTmpLen = std::max(scan_tmp_len(sh.nKeys), scan_tmp_len(slots));
+    auto alloc = [&](void **q, size_t bytes) {
+        return hipMalloc(q, std::max<size_t>(bytes, 256)) == hipSuccess;
+    };
+    bool ok = alloc((void **)&p->d_sSize, sizeof(int) * g->nPlanes) &&
+              alloc((void **)&p->entries, sizeof(int4) * sh.maxEntries) &&
+              alloc((void **)&p->items, sizeof(int4) * p->maxItems) &&
+              alloc((void **)&p->counts, sizeof(int) * (sh.nKeys + 1)) &&
+              alloc((void **)&p->starts, sizeof(int) * (sh.nKeys + 1)) &&
+              alloc((void **)&p->cursor, sizeof(int) * (sh.nKeys + 1)) &&
+              alloc((void **)&p->itemCount, sizeof(int) * (slots + 1)) &&
+              alloc((void **)&p->itemStart, sizeof(int) * (slots + 1)) &&
+              alloc((void **)&p->meta, sizeof(int) * 8) &&
+              alloc((void **)&p->scanTmp, sizeof(int) * p->scanTmpLen);
+    if (!ok) {
+        (void)hipGetLastError();
+        free_plan(p);
+        delete p;
+        return tcv_fail(ctx, TCV_ERR_NOMEM, "plan allocation failed");
+    }
+    if (hipMemcpy(p->d_sSize, g->sSize, sizeof(int) * g->nPlanes, hipMemcpyHostToDevice) != hipSuccess) {
+        free_plan(p);
+        delete p;
+        return tcv_fail(ctx, TCV_ERR_DEVICE, "plan upload failed");
+    }
+    st = tcv_plan_build(ctx, p, g);
+    if (st == TCV_OK) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st = tcv_fail(ctx, TCV_ERR_DEVICE, hipGetErrorString(e));
+    }
+    if (st != TCV_OK) {
+        free_plan(p);
+        delete p;
+        return st;
+    }
+    *out = p;
+    return TCV_OK;
+}
+
+tcv_status tcv_plan_build(tcv_ctx *ctx, tcv_plan *p, const tcv_geometry *g)
+{
+    int maxS = 0;
+    tcv_status st = check_geometry(ctx, g, &maxS);
+    if (st != TCV_OK) return st;
+    if (!p) return tcv_fail(ctx, TCV_ERR_INVALID, "null plan");
+    if (g->nVis > p->nVis || g->gSize != p->gSize || g->nPlanes != p->nPlanes || maxS > p->sh.maxS ||
+        g->cSize > p->cSize ||
+        !std::equal(g->sSize, g->sSize + g->nPlanes, p->sSizeHost.begin()))
+        return tcv_fail(ctx, TCV_ERR_INVALID, "geometry does not match the plan");
+    GeomDev gd;
+    gd.nVis = g->nVis;
+    gd.iu = g->d_iu;
+    gd.iv = g->d_iv;
+    gd.cOffset = g->d_cOffset;
+    gd.wPlane = g->d_wPlane;
+    gd.nPlanes = g->nPlanes;
+    gd.sUniform = g->sSize[0];
+    gd.gSize = g->gSize;
+    gd.cSize = g->cSize;
+    gd.sSize = g->d_wPlane ? p->d_sSize : nullptr;
+    p->cSize = g->cSize;
+    return plan_build_kernels(ctx, p, gd);
+}
+
+tcv_status tcv_plan_destroy(tcv_ctx *ctx, tcv_plan *p)
+{
+    if (!ctx) return TCV_ERR_INVALID;
+    if (!p) return TCV_OK;
+    TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    free_plan(p);
+    delete p;
+    return TCV_OK;
+}
+
+tcv_status tcv_plan_info(tcv_ctx *ctx, tcv_plan *p, tcv_tuning *t, long long *entries,
+                         long long *items, long long *outOfRange)
+{
+    if (!ctx || !p) return TCV_ERR_INVALID;
+    if (t) *t = tcv_tuning{p->sh.tw, p->sh.th, p->sh.chunk, p->sh.k2, p->sh.phases};
+    int meta[4] = {0, 0, 0, 0};
+    TCV_HIP(ctx, hipMemcpyAsync(meta, p->meta, sizeof(meta), hipMemcpyDeviceToHost, ctx->stream));
+    TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (items) *items = meta[0];
+    if (entries) *entries = meta[2];
+    if (outOfRange) *outOfRange = meta[3];
+    return TCV_OK;
+}
+
+tcv_status tcv_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const float *d_data, const float *d_C,
+                          float *d_grid)
+{
+    if (!ctx || !p || !d_data || !d_C || !d_grid) return tcv_fail(ctx, TCV_ERR_INVALID, "null argument");
+    return launch_grid_tiled(ctx, p, (const float2 *)d_data, (const float2 *)d_C, (float2 *)d_grid);
+}
+
+tcv_status tcv_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float *d_grid, const float *d_C,
+                            float *d_data)
+{
+    if (!ctx || !p || !d_data || !d_C || !d_grid) return tcv_fail(ctx, TCV_ERR_INVALID, "null argument");
+    return launch_degrid_tiled(ctx, p, (const float2 *)d_grid, (const float2 *)d_C, (float2 *)d_data);
+}
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+// Internal declarations shared by the libtcv translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "tcv.h"
+
+struct tcv_ctx {
+    int device = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    hipEvent_t timer0 = nullptr, timer1 = nullptr;
+    bool profiling = false;
+    struct Pending { hipEvent_t a, b; std::string family; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    struct Stat { long launches = 0; double ms = 0.0; };
+    std::map<std::string, Stat> stats;
+    // device counters: [0] visibilities skipped by the plain kernels (range)
+    unsigned long long *d_counters = nullptr;
+};
+
+tcv_status tcv_fail(tcv_ctx *ctx, tcv_status s, const std::string &msg);
+
+#define TCV_HIP(ctx, expr)                                                      \
+    do {                                                                        \
+        hipError_t e_ = (expr);                                                 \
+        if (e_ != hipSuccess)                                                   \
+            return tcv_fail((ctx), TCV_ERR_DEVICE,                              \
+                            std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Brackets kernel launches with events when profiling is on.
+struct ProfScope {
+    tcv_ctx *ctx;
+    hipEvent_t a = nullptr, b = nullptr;
+    const char *family;
+    ProfScope(tcv_ctx *c, const char *fam);
+    ~ProfScope();
+};
+
+// Device-side view of a job's geometry (sSize copied to device memory).
+struct GeomDev {
+    int nVis;
+    const int *iu, *iv, *cOffset, *wPlane;
+    const int *sSize;  // device copy, nPlanes entries (nullptr when uniform)
+    int nPlanes;
+    int sUniform;      // kernel width when wPlane == nullptr
+    int gSize;
+    long long cSize;
+};
+
+// Launchers implemented in tcv_kernels.hip.  All enqueue on ctx->stream.
+tcv_status launch_fill_u32(tcv_ctx *ctx, void *p, unsigned value, size_t n32);
+tcv_status launch_offsets(tcv_ctx *ctx, const double *u, const double *v,
+                          const double *w, int nSamples, const double *d_freq,
+                          int nChan, double cellSize, double wCellSize, int wSize,
+                          int gSize, int overSample, const int *d_sSize,
+                          const int *d_cOffset0, int *iu, int *iv, int *wPlane,
+                          int *cOffset, unsigned long long *d_numPerPlane,
+                          unsigned long long *d_bad);
+tcv_status launch_grid_plain(tcv_ctx *ctx, const GeomDev &g, int maxS,
+                             const float2 *data, const float2 *C, float2 *grid);
+tcv_status launch_degrid_plain(tcv_ctx *ctx, const GeomDev &g,
+                               const float2 *grid, const float2 *C, float2 *out);
+
+// ---- tiled path -------------------------------------------------------------
+
+struct PlanShape {
+    int tw, th, chunk, k2, phases;
+    int nTX, nTY, nTiles;
+    int maxS;
+    long long maxEntries;  // upper bound on visibility x tile pairs
+    long long nKeys;       // nTiles * k2
+};
+
+struct tcv_plan {
+    PlanShape sh{};
+    int nVis = 0, gSize = 0, nPlanes = 0;
+    std::vector<int> sSizeHost;
+    int *d_sSize = nullptr;
+    int4 *entries = nullptr;      // maxEntries
+    int4 *items = nullptr;        // maxItems
+    long long maxItems = 0;
+    int *counts = nullptr;        // nKeys + 1
+    int *starts = nullptr;        // nKeys + 1
+    int *cursor = nullptr;        // nKeys
+    int *itemCount = nullptr;     // phases * nTiles + 1
+    int *itemStart = nullptr;     // phases * nTiles + 1
+    int *meta = nullptr;          // [0] items [1] queue head [2] entries [3] bad
+    int *scanTmp = nullptr;       // scratch for the multi-level scan
+    long long scanTmpLen = 0;
+    long long cSize = 0;
+};
+
+tcv_status plan_build_kernels(tcv_ctx *ctx, tcv_plan *p, const GeomDev &g);
+tcv_status launch_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *data,
+                             const float2 *C, float2 *grid);
+tcv_status launch_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *grid,
+                               const float2 *C, float2 *out);
+long long scan_tmp_len(long long n);
+// Supported tile configurations (tile_w, tile_h) of the tiled kernels.
+bool tiled_config_supported(int tw, int th);

@@ -1,0 +1,808 @@
+// HIP kernels for gfx950 (MI355X): FP64 index set-up, plain and tiled
+// gridding/degridding, and the device-side binning plan.
+//
+// Design notes (the reference loops these replace are cited per kernel):
+//  * wave64 everywhere; no MFMA -- gridding is a scatter-add, degridding a
+//    gather-reduce, both bound by the C(w) stream (8 B per grid point).
+//  * C(w) is read through raw buffer loads with a 32-bit byte offset; lanes
+//    with nothing to do get an offset past num_records, which the buffer unit
+//    returns as zero without touching memory, so masked lanes need no exec
+//    juggling and can never read out of bounds.
+//  * The tiled kernels keep the grid tile in VGPRs (owner computes) and
+//    stream only C; the plain kernels are the one-atomic-per-point baseline.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <type_traits>
+
+#include "tcv_internal.h"
+
+namespace {
+
+constexpr unsigned kOOB = 0x80000000u;  // byte offset beyond any C buffer we accept
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, unsigned bytes)
+{
+    // word3 0x00020000: raw 32-bit buffer format used for untyped loads on gfx9.
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float2 load_c(__amdgpu_buffer_rsrc_t r, unsigned voff)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
+}
+
+
+// Sum over the 64 lanes of a wave (all lanes must be active).  Four DPP
+// steps reduce each 16-lane row; four readlanes combine the rows.  The result
+// is wave-uniform.
+__device__ __forceinline__ float wave_sum(float x)
+{
+    auto dpp = [](float v, auto ctrl) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+            0, __builtin_bit_cast(int, v), decltype(ctrl)::value, 0xF, 0xF, false));
+    };
+    x += dpp(x, std::integral_constant<int, 0xB1>{});   // quad_perm [1,0,3,2]
+    x += dpp(x, std::integral_constant<int, 0x4E>{});   // quad_perm [2,3,0,1]
+    x += dpp(x, std::integral_constant<int, 0x141>{});  // row_half_mirror
+    x += dpp(x, std::integral_constant<int, 0x140>{});  // row_mirror
+    const int xi = __builtin_bit_cast(int, x);
+    const float r0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(xi, 0));
+    const float r1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(xi, 16));
+    const float r2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(xi, 32));
+    const float r3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(xi, 48));
+    return (r0 + r1) + (r2 + r3);
+}
+
+// ---- fill (graph-safe zeroing) ------------------------------------------------
+
+__global__ void k_fill_u32(unsigned *p, unsigned v, size_t n)
+{
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+// ---- a6: FP64 index set-up ----------------------------------------------------
+// attic/tConvolveMPI/Benchmark.cc:293-322 and current/tConvolveMPI/Benchmark.cc:610-643.
+// Same expression order as the reference; contraction off so every product and
+// quotient is rounded exactly as on the host.
+__global__ __launch_bounds__(256) void k_offsets(
+    const double *__restrict__ u, const double *__restrict__ v,
+    const double *__restrict__ w, int nSamples, const double *__restrict__ freq,
+    int nChan, double cellSize, double wCellSize, int wSize, int gSize,
+    int overSample, const int *__restrict__ sSize, const int *__restrict__ cOffset0,
+    int *__restrict__ iuOut, int *__restrict__ ivOut, int *__restrict__ wPlaneOut,
+    int *__restrict__ cOffsetOut, unsigned long long *__restrict__ numPerPlane,
+    unsigned long long *__restrict__ bad, int useLdsHist)
+{
+#pragma clang fp contract(off)
+    extern __shared__ unsigned hist[];
+    if (useLdsHist) {
+        for (int k = threadIdx.x; k < wSize; k += blockDim.x) hist[k] = 0;
+        __syncthreads();
+    }
+    const int total = nSamples * nChan;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int isBad = 0;
+    if (idx < total) {
+        const int i = idx / nChan;
+        const int chan = idx - i * nChan;
+        const double f = freq[chan];
+
+        const double uScaled = f * u[i] / cellSize;
+        int a = (int)uScaled;
+        if (uScaled < (double)a) a -= 1;
+        const int fracu = (int)(overSample * (uScaled - (double)a));
+        const int iu = a + gSize / 2;
+
+        const double vScaled = f * v[i] / cellSize;
+        int b = (int)vScaled;
+        if (vScaled < (double)b) b -= 1;
+        const int fracv = (int)(overSample * (vScaled - (double)b));
+        const int iv = b + gSize / 2;
+
+        int woff;
+        int s, co;
+        if (cOffset0 == nullptr) {
+            // attic: uniform width, planes laid out contiguously
+            const double wScaled = f * w[i] / wCellSize;
+            woff = wSize / 2 + (int)wScaled;
+            s = sSize[0];
+            co = s * s * (fracu + overSample * (fracv + overSample * woff));
+        } else {
+            woff = 0;
+            if (wCellSize > 0.0) {
+                const double wScaled = f * w[i] / wCellSize;
+                woff = wSize / 2 + (int)wScaled;
+            }
+            const bool planeOk = woff >= 0 && woff < wSize;
+            s = planeOk ? sSize[woff] : 0;
+            co = planeOk ? s * s * (fracu + overSample * fracv) + cOffset0[woff] : 0;
+        }
+        const bool planeOk = woff >= 0 && woff < wSize;
+        const int half = s / 2;
+        // footprint rows [iv, iv + s), columns [iu - s/2, iu - s/2 + s)
+        isBad = !planeOk || iu - half < 0 || iv < 0 || iu - half + s > gSize ||
+                iv + s > gSize;
+        iuOut[idx] = iu;
+        ivOut[idx] = iv;
+        cOffsetOut[idx] = co;
+        if (wPlaneOut) wPlaneOut[idx] = woff;
+        if (numPerPlane && planeOk) {
+            if (useLdsHist) atomicAdd(&hist[woff], 1u);
+            else atomicAdd(&numPerPlane[woff], 1ull);
+        }
+    }
+    // one atomic per wave for the out-of-range count
+    const unsigned long long nb = __builtin_popcountll(__ballot(isBad));
+    if ((threadIdx.x & 63) == 0 && nb) atomicAdd(bad, nb);
+    if (useLdsHist) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < wSize; k += blockDim.x)
+            if (hist[k]) atomicAdd(&numPerPlane[k], (unsigned long long)hist[k]);
+    }
+}
+
+// Validates one visibility; returns its width and footprint corner.
+struct VisGeom {
+    int s, u0, v0, c0;
+    bool ok;
+};
+
+__device__ __forceinline__ VisGeom vis_geom(const GeomDev &g, int d)
+{
+    VisGeom r;
+    int s = g.sUniform;
+    bool ok = true;
+    if (g.wPlane) {
+        const int p = g.wPlane[d];
+        ok = p >= 0 && p < g.nPlanes;
+        s = ok ? g.sSize[p] : 0;
+    }
+    // footprint rows [iv, iv + s), columns [iu - s/2, iu - s/2 + s): the
+    // reference offsets only the column (gind = iu + gSize*iv - support,
+    // attic/tConvolveMPI/Benchmark.cc:133)
+    const int half = s / 2;
+    r.s = s;
+    r.u0 = g.iu[d] - half;
+    r.v0 = g.iv[d];
+    r.c0 = g.cOffset[d];
+    ok = ok && s > 0 && r.u0 >= 0 && r.v0 >= 0 && r.u0 + s <= g.gSize &&
+         r.v0 + s <= g.gSize && r.c0 >= 0 &&
+         (long long)r.c0 + (long long)s * s <= g.cSize;
+    r.ok = ok;
+    return r;
+}
+
+// ---- a7 plain gridder ---------------------------------------------------------
+// attic/tConvolveMPI/Benchmark.cc:131-154: one workgroup per visibility, its
+// threads walk the flattened sSize x sSize footprint; every tap is one pair of
+// FP32 global atomics (the one-job-per-launch baseline).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_grid_plain(GeomDev g, const float2 *__restrict__ data,
+                                                   const float2 *__restrict__ C,
+                                                   float2 *__restrict__ grid,
+                                                   unsigned long long *bad)
+{
+    const int d = blockIdx.x;
+    const VisGeom vg = vis_geom(g, d);
+    if (!vg.ok) {
+        if (threadIdx.x == 0) atomicAdd(bad, 1ull);
+        return;
+    }
+    const int s = vg.s;
+    const float2 dv = data[d];
+    const int n = s * s;
+    int r = threadIdx.x / s, c = threadIdx.x - (threadIdx.x / s) * s;
+    const int sr = BS / s, sc = BS - (BS / s) * s;
+    const float2 *cp = C + vg.c0;
+    for (int p = threadIdx.x; p < n; p += BS) {
+        const float2 cv = cp[p];
+        float *gp = reinterpret_cast<float *>(grid + ((size_t)(vg.v0 + r) * g.gSize + vg.u0 + c));
+        unsafeAtomicAdd(gp, dv.x * cv.x - dv.y * cv.y);
+        unsafeAtomicAdd(gp + 1, dv.x * cv.y + dv.y * cv.x);
+        c += sc;
+        r += sr;
+        if (c >= s) { c -= s; r++; }
+    }
+}
+
+// ---- a8 plain degridder -------------------------------------------------------
+// attic/tConvolveMPI/Benchmark.cc:165-193: one wave64 per visibility, lanes walk
+// the flattened footprint, then a DPP reduction.
+__global__ __launch_bounds__(256) void k_degrid_plain(GeomDev g, const float2 *__restrict__ grid,
+                                                      const float2 *__restrict__ C,
+                                                      float2 *__restrict__ out,
+                                                      unsigned long long *bad)
+{
+    const int lane = threadIdx.x & 63;
+    const int d = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (d >= g.nVis) return;   // whole wave leaves together
+    const VisGeom vg = vis_geom(g, d);
+    if (!vg.ok) {
+        if (lane == 0) {
+            atomicAdd(bad, 1ull);
+            out[d] = make_float2(0.f, 0.f);
+        }
+        return;
+    }
+    const int s = vg.s;
+    const int n = s * s;
+    int r = lane / s, c = lane - (lane / s) * s;
+    const int sr = 64 / s, sc = 64 - (64 / s) * s;
+    const float2 *cp = C + vg.c0;
+    float re = 0.f, im = 0.f;
+    for (int p = lane; p < n; p += 64) {
+        const float2 cv = cp[p];
+        const float2 gv = grid[(size_t)(vg.v0 + r) * g.gSize + vg.u0 + c];
+        re += gv.x * cv.x - gv.y * cv.y;
+        im += gv.x * cv.y + gv.y * cv.x;
+        c += sc;
+        r += sr;
+        if (c >= s) { c -= s; r++; }
+    }
+    re = wave_sum(re);
+    im = wave_sum(im);
+    if (lane == 0) out[d] = make_float2(re, im);
+}
+
+// ---- scan (exclusive, int32, total written at out[n]) -------------------------
+
+constexpr int SCAN_BS = 256;
+constexpr int SCAN_IPT = 16;
+constexpr int SCAN_TILE = SCAN_BS * SCAN_IPT;
+
+__device__ __forceinline__ int block_exclusive_scan(int x, int *total)
+{
+    __shared__ int wsum[SCAN_BS / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int incl = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int before = 0, all = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_BS / 64; ++k) {
+        const int t = wsum[k];
+        if (k < wv) before += t;
+        all += t;
+    }
+    __syncthreads();
+    *total = all;
+    return before + incl - x;
+}
+
+__global__ __launch_bounds__(SCAN_BS) void k_scan_reduce(const int *__restrict__ in, long long n,
+                                                         int *__restrict__ sums)
+{
+    const long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_IPT;
+    int t = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; ++k)
+        if (base + k < n) t += in[base + k];
+    int total;
+    (void)block_exclusive_scan(t, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SCAN_BS) void k_scan_apply(const int *__restrict__ in, long long n,
+                                                        const int *__restrict__ offs,
+                                                        int *__restrict__ out)
+{
+    const long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_IPT;
+    int vals[SCAN_IPT];
+    int t = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; ++k) {
+        vals[k] = (base + k < n) ? in[base + k] : 0;
+        t += vals[k];
+    }
+    int total;
+    int run = block_exclusive_scan(t, &total) + (offs ? offs[blockIdx.x] : 0);
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += vals[k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_BS - 1) out[n] = run;
+}
+
+// ---- tiled path: plan kernels -------------------------------------------------
+
+struct PlanDevArgs {
+    int tw, th, nTX, k2, phases, chunk, nTiles;
+    long long cSize;
+};
+
+// pass 0 counts (tile, slice-key) buckets; pass 1 scatters the entries.
+// A visibility with footprint [u0, u0+s) x [v0, v0+s) is entered once in every
+// tile its footprint overlaps.  Entry = {u0 | v0 << 16, cOffset, vis, s}.
+__global__ __launch_bounds__(256) void k_plan_bin(int pass, GeomDev g, PlanDevArgs a,
+                                                  int *__restrict__ counts,
+                                                  int *__restrict__ cursor,
+                                                  int4 *__restrict__ entries,
+                                                  int *__restrict__ meta)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    int isBad = 0;
+    if (d < g.nVis) {
+        const VisGeom vg = vis_geom(g, d);
+        if (!vg.ok) {
+            isBad = 1;
+        } else {
+            const int key2 = (int)(((long long)vg.c0 * a.k2) / a.cSize);
+            const int tx0 = vg.u0 / a.tw, tx1 = (vg.u0 + vg.s - 1) / a.tw;
+            const int ty0 = vg.v0 / a.th, ty1 = (vg.v0 + vg.s - 1) / a.th;
+            const int4 ent = make_int4(vg.u0 | (vg.v0 << 16), vg.c0, d, vg.s);
+            for (int ty = ty0; ty <= ty1; ++ty)
+                for (int tx = tx0; tx <= tx1; ++tx) {
+                    const long long key = (long long)(ty * a.nTX + tx) * a.k2 + key2;
+                    if (pass == 0) {
+                        atomicAdd(&counts[key], 1);
+                    } else {
+                        const int pos = atomicAdd(&cursor[key], 1);
+                        entries[pos] = ent;
+                    }
+                }
+        }
+    }
+    if (pass == 0) {
+        const int nb = __builtin_popcountll(__ballot(isBad));
+        if ((threadIdx.x & 63) == 0 && nb) atomicAdd(&meta[3], nb);
+    }
+}
+
+// Work items: each (phase, tile) range of entries is cut into chunks of at most
+// `chunk` entries.  Items are ordered phase-major so concurrently running
+// workgroups read neighbouring C slices.  Item = {tile, first, count, exclusive}.
+__global__ __launch_bounds__(256) void k_plan_items(int pass, PlanDevArgs a,
+                                                    const int *__restrict__ starts,
+                                                    int *__restrict__ itemCount,
+                                                    const int *__restrict__ itemStart,
+                                                    int4 *__restrict__ items)
+{
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= a.phases * a.nTiles) return;
+    const int p = id / a.nTiles, t = id - p * a.nTiles;
+    const int k0 = (int)((long long)p * a.k2 / a.phases);
+    const int k1 = (int)((long long)(p + 1) * a.k2 / a.phases);
+    const long long b0 = (long long)t * a.k2;
+    const int s = starts[b0 + k0], e = starts[b0 + k1];
+    const int n = e - s;
+    const int nc = (n + a.chunk - 1) / a.chunk;
+    if (pass == 0) {
+        itemCount[id] = nc;
+    } else {
+        const int excl = (a.phases == 1 && nc == 1) ? 1 : 0;
+        const int o = itemStart[id];
+        for (int k = 0; k < nc; ++k) {
+            const int f = s + k * a.chunk;
+            items[o + k] = make_int4(t, f, min(a.chunk, e - f), excl);
+        }
+    }
+}
+
+__global__ void k_plan_meta(const int *__restrict__ itemStart, int nItemSlots,
+                            const int *__restrict__ starts, long long nKeys,
+                            int *__restrict__ meta)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        meta[0] = itemStart[nItemSlots];
+        meta[1] = 0;
+        meta[2] = starts[nKeys];
+    }
+}
+
+__global__ void k_reset_queue(int *meta) { if (threadIdx.x == 0) meta[1] = 0; }
+
+// ---- tiled gridder ------------------------------------------------------------
+// attic/tConvolveMPI/Benchmark.cc:131-154 reorganised as owner-computes: a
+// workgroup owns a TW x TH grid tile held in VGPRs.  Lane l covers column
+// l % TW and, when TW < 64, row l / TW of each 64/TW-row group; wave w owns R
+// consecutive row groups.  For every binned visibility the workgroup streams
+// the part of its C slice that falls on the tile, accumulates, and writes the
+// tile back once.  Persistent: workgroups pull items from a device queue.
+template <int TW, int R, int NW, int RB>
+__global__ __launch_bounds__(NW * 64) void k_grid_tiled(
+    const int4 *__restrict__ items, const int4 *__restrict__ entries,
+    int *__restrict__ meta, int nTX, const float2 *__restrict__ data,
+    const float2 *__restrict__ C, unsigned cBytes, float2 *__restrict__ grid, int gSize)
+{
+    constexpr int RPI = 64 / TW;
+    constexpr int TH = NW * R * RPI;
+    static_assert(R % RB == 0, "row block must divide the register rows");
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lx = lane % TW, ly = lane / TW;
+    const __amdgpu_buffer_rsrc_t rc = make_rsrc(C, cBytes);
+    __shared__ int s_item;
+    const int nItems = __builtin_amdgcn_readfirstlane(meta[0]);
+    const int q0 = wave * R;   // first row group of this wave
+
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(&meta[1], 1);
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (item >= nItems) break;
+        const int4 it = items[item];
+        const int tyi = it.x / nTX, txi = it.x - tyi * nTX;
+        const int Bx = txi * TW, By = tyi * TH;
+        const int e0 = it.y, e1 = it.y + it.z;
+
+        float ar[R], ai[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) { ar[j] = 0.f; ai[j] = 0.f; }
+
+        int4 enA = entries[e0];
+        int4 enB = entries[min(e0 + 1, e1 - 1)];
+        float2 dA = data[enA.z];
+        for (int e = e0; e < e1; ++e) {
+            const float2 dB = data[enB.z];
+            const int4 enC = entries[min(e + 2, e1 - 1)];
+
+            const int u0 = enA.x & 0xFFFF, v0 = (int)((unsigned)enA.x >> 16);
+            const int c0 = enA.y, s = enA.w;
+            const int cx = Bx + lx - u0;
+            const int dy = By - v0;   // C row of tile row r is r + dy
+            const unsigned lanev = ((unsigned)cx < (unsigned)s)
+                                       ? (unsigned)(c0 + cx) * 8u : kOOB;
+#pragma unroll
+            for (int jb = 0; jb < R; jb += RB) {
+                const int r0 = (q0 + jb) * RPI;
+                if (r0 + RB * RPI + dy > 0 && r0 + dy < s) {
+                    float2 cv[RB];
+#pragma unroll
+                    for (int k = 0; k < RB; ++k) {
+                        const int cy = r0 + k * RPI + ly + dy;
+                        const unsigned vo = ((unsigned)cy < (unsigned)s)
+                                                ? lanev + (unsigned)(cy * s) * 8u : kOOB;
+                        cv[k] = load_c(rc, vo);
+                    }
+#pragma unroll
+                    for (int k = 0; k < RB; ++k) {
+                        ar[jb + k] = fmaf(dA.x, cv[k].x, fmaf(-dA.y, cv[k].y, ar[jb + k]));
+                        ai[jb + k] = fmaf(dA.x, cv[k].y, fmaf(dA.y, cv[k].x, ai[jb + k]));
+                    }
+                }
+            }
+            enA = enB;
+            dA = dB;
+            enB = enC;
+        }
+
+        const int col = Bx + lx;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int row = By + (q0 + j) * RPI + ly;
+            if (row < gSize && col < gSize && (ar[j] != 0.f || ai[j] != 0.f)) {
+                float2 *gp = grid + (size_t)row * gSize + col;
+                if (it.w) {
+                    float2 o = *gp;
+                    o.x += ar[j];
+                    o.y += ai[j];
+                    *gp = o;
+                } else {
+                    unsafeAtomicAdd(&gp->x, ar[j]);
+                    unsafeAtomicAdd(&gp->y, ai[j]);
+                }
+            }
+        }
+    }
+}
+
+// ---- tiled degridder ----------------------------------------------------------
+// attic/tConvolveMPI/Benchmark.cc:165-193 reorganised: every wave holds the
+// whole TW x TH grid tile in VGPRs (read once per work item); the waves of a
+// workgroup split the tile's visibilities.  Each visibility's partial sum over
+// the tile is reduced across the wave with DPP and added to the output with
+// two FP32 atomics (the output is zeroed first: degridding overwrites).
+template <int TW, int RD, int NW, int RB>
+__global__ __launch_bounds__(NW * 64) void k_degrid_tiled(
+    const int4 *__restrict__ items, const int4 *__restrict__ entries,
+    int *__restrict__ meta, int nTX, const float2 *__restrict__ grid, int gSize,
+    const float2 *__restrict__ C, unsigned cBytes, float2 *__restrict__ out)
+{
+    constexpr int RPI = 64 / TW;
+    constexpr int TH = RD * RPI;
+    static_assert(RD % RB == 0, "row block must divide the register rows");
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lx = lane % TW, ly = lane / TW;
+    const __amdgpu_buffer_rsrc_t rc = make_rsrc(C, cBytes);
+    __shared__ int s_item;
+    const int nItems = __builtin_amdgcn_readfirstlane(meta[0]);
+
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(&meta[1], 1);
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (item >= nItems) break;
+        const int4 it = items[item];
+        const int tyi = it.x / nTX, txi = it.x - tyi * nTX;
+        const int Bx = txi * TW, By = tyi * TH;
+        const int e0 = it.y, e1 = it.y + it.z;
+        const int col = Bx + lx;
+
+        float2 gv[RD];
+#pragma unroll
+        for (int j = 0; j < RD; ++j) {
+            const int row = By + j * RPI + ly;
+            gv[j] = (row < gSize && col < gSize) ? grid[(size_t)row * gSize + col]
+                                                  : make_float2(0.f, 0.f);
+        }
+
+        const int first = e0 + wave;
+        if (first >= e1) continue;
+        int4 enA = entries[first];
+        for (int e = first; e < e1; e += NW) {
+            const int4 enB = entries[min(e + NW, e1 - 1)];
+            const int u0 = enA.x & 0xFFFF, v0 = (int)((unsigned)enA.x >> 16);
+            const int c0 = enA.y, s = enA.w;
+            const int cx = Bx + lx - u0;
+            const int dy = By - v0;
+            const unsigned lanev = ((unsigned)cx < (unsigned)s)
+                                       ? (unsigned)(c0 + cx) * 8u : kOOB;
+            float sr = 0.f, si = 0.f;
+#pragma unroll
+            for (int jb = 0; jb < RD; jb += RB) {
+                const int r0 = jb * RPI;
+                if (r0 + RB * RPI + dy > 0 && r0 + dy < s) {
+                    float2 cv[RB];
+#pragma unroll
+                    for (int k = 0; k < RB; ++k) {
+                        const int cy = r0 + k * RPI + ly + dy;
+                        const unsigned vo = ((unsigned)cy < (unsigned)s)
+                                                ? lanev + (unsigned)(cy * s) * 8u : kOOB;
+                        cv[k] = load_c(rc, vo);
+                    }
+#pragma unroll
+                    for (int k = 0; k < RB; ++k) {
+                        sr = fmaf(gv[jb + k].x, cv[k].x, fmaf(-gv[jb + k].y, cv[k].y, sr));
+                        si = fmaf(gv[jb + k].x, cv[k].y, fmaf(gv[jb + k].y, cv[k].x, si));
+                    }
+                }
+            }
+            sr = wave_sum(sr);
+            si = wave_sum(si);
+            if (lane == 0) {
+                float *op = reinterpret_cast<float *>(out + enA.z);
+                unsafeAtomicAdd(op, sr);
+                unsafeAtomicAdd(op + 1, si);
+            }
+            enA = enB;
+        }
+    }
+}
+
+int grid_blocks(size_t n, int bs)
+{
+    size_t b = (n + bs - 1) / bs;
+    return (int)std::min<size_t>(b, 1u << 20);
+}
+
+}  // namespace
+
+// ---- launchers ----------------------------------------------------------------
+
+tcv_status launch_fill_u32(tcv_ctx *ctx, void *p, unsigned value, size_t n32)
+{
+    if (n32 == 0) return TCV_OK;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_blocks(n32, 256)), dim3(256), 0, ctx->stream,
+                       (unsigned *)p, value, n32);
+    TCV_HIP(ctx, hipGetLastError());
+    return TCV_OK;
+}
+
+tcv_status launch_offsets(tcv_ctx *ctx, const double *u, const double *v,
+                          const double *w, int nSamples, const double *d_freq,
+                          int nChan, double cellSize, double wCellSize, int wSize,
+                          int gSize, int overSample, const int *d_sSize,
+                          const int *d_cOffset0, int *iu, int *iv, int *wPlane,
+                          int *cOffset, unsigned long long *d_numPerPlane,
+                          unsigned long long *d_bad)
+{
+    const int total = nSamples * nChan;
+    if (total == 0) return TCV_OK;
+    const int useLds = (d_numPerPlane && wSize <= 8192) ? 1 : 0;
+    const size_t lds = useLds ? (size_t)wSize * 4 : 0;
+    ProfScope ps(ctx, "offsets");
+    hipLaunchKernelGGL(k_offsets, dim3((total + 255) / 256), dim3(256), lds, ctx->stream,
+                       u, v, w, nSamples, d_freq, nChan, cellSize, wCellSize, wSize,
+                       gSize, overSample, d_sSize, d_cOffset0, iu, iv, wPlane, cOffset,
+                       d_numPerPlane, d_bad, useLds);
+    TCV_HIP(ctx, hipGetLastError());
+    return TCV_OK;
+}
+
+tcv_status launch_grid_plain(tcv_ctx *ctx, const GeomDev &g, int maxS,
+                             const float2 *data, const float2 *C, float2 *grid)
+{
+    if (g.nVis == 0) return TCV_OK;
+    ProfScope ps(ctx, "grid_plain");
+    if (maxS * maxS <= 64)
+        hipLaunchKernelGGL(k_grid_plain<64>, dim3(g.nVis), dim3(64), 0, ctx->stream, g, data,
+                           C, grid, ctx->d_counters);
+    else
+        hipLaunchKernelGGL(k_grid_plain<256>, dim3(g.nVis), dim3(256), 0, ctx->stream, g, data,
+                           C, grid, ctx->d_counters);
+    TCV_HIP(ctx, hipGetLastError());
+    return TCV_OK;
+}
+
+tcv_status launch_degrid_plain(tcv_ctx *ctx, const GeomDev &g, const float2 *grid,
+                               const float2 *C, float2 *out)
+{
+    if (g.nVis == 0) return TCV_OK;
+    ProfScope ps(ctx, "degrid_plain");
+    hipLaunchKernelGGL(k_degrid_plain, dim3((g.nVis + 3) / 4), dim3(256), 0, ctx->stream, g,
+                       grid, C, out, ctx->d_counters);
+    TCV_HIP(ctx, hipGetLastError());
+    return TCV_OK;
+}
+
+long long scan_tmp_len(long long n)
+{
+    // sums + scanned sums for every level
+    long long len = 0;
+    while (true) {
+        const long long nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+        if (nb <= 1) break;
+        len += 2 * (nb + 1);
+        n = nb;
+    }
+    return len + 2;
+}
+
+static tcv_status scan_exclusive(tcv_ctx *ctx, const int *in, int *out, long long n, int *tmp)
+{
+    const long long nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb <= 1) {
+        hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(SCAN_BS), 0, ctx->stream, in, n,
+                           (const int *)nullptr, out);
+        TCV_HIP(ctx, hipGetLastError());
+        return TCV_OK;
+    }
+    int *sums = tmp;
+    int *sumsScan = tmp + nb + 1;
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_BS), 0, ctx->stream, in, n,
+                       sums);
+    TCV_HIP(ctx, hipGetLastError());
+    tcv_status st = scan_exclusive(ctx, sums, sumsScan, nb, tmp + 2 * (nb + 1));
+    if (st != TCV_OK) return st;
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SCAN_BS), 0, ctx->stream, in, n,
+                       (const int *)sumsScan, out);
+    TCV_HIP(ctx, hipGetLastError());
+    return TCV_OK;
+}
+
+tcv_status plan_build_kernels(tcv_ctx *ctx, tcv_plan *p, const GeomDev &g)
+{
+    const PlanShape &sh = p->sh;
+    PlanDevArgs a{sh.tw, sh.th, sh.nTX, sh.k2, sh.phases, sh.chunk, sh.nTiles, g.cSize};
+    ProfScope ps(ctx, "plan");
+    tcv_status st;
+    if ((st = launch_fill_u32(ctx, p->counts, 0, (size_t)sh.nKeys + 1)) != TCV_OK) return st;
+    if ((st = launch_fill_u32(ctx, p->meta, 0, 8)) != TCV_OK) return st;
+    const int nb = (g.nVis + 255) / 256;
+    if (g.nVis > 0) {
+        hipLaunchKernelGGL(k_plan_bin, dim3(nb), dim3(256), 0, ctx->stream, 0, g, a, p->counts,
+                           p->cursor, p->entries, p->meta);
+        TCV_HIP(ctx, hipGetLastError());
+    }
+    if ((st = scan_exclusive(ctx, p->counts, p->starts, sh.nKeys, p->scanTmp)) != TCV_OK)
+        return st;
+    TCV_HIP(ctx, hipMemcpyAsync(p->cursor, p->starts, sizeof(int) * sh.nKeys,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+    if (g.nVis > 0) {
+        hipLaunchKernelGGL(k_plan_bin, dim3(nb), dim3(256), 0, ctx->stream, 1, g, a, p->counts,
+                           p->cursor, p->entries, p->meta);
+        TCV_HIP(ctx, hipGetLastError());
+    }
+    const int slots = sh.phases * sh.nTiles;
+    hipLaunchKernelGGL(k_plan_items, dim3((slots + 255) / 256), dim3(256), 0, ctx->stream, 0, a,
+                       p->starts, p->itemCount, (const int *)p->itemStart, p->items);
+    TCV_HIP(ctx, hipGetLastError());
+    if ((st = scan_exclusive(ctx, p->itemCount, p->itemStart, slots, p->scanTmp)) != TCV_OK)
+        return st;
+    hipLaunchKernelGGL(k_plan_items, dim3((slots + 255) / 256), dim3(256), 0, ctx->stream, 1, a,
+                       p->starts, p->itemCount, (const int *)p->itemStart, p->items);
+    TCV_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_plan_meta, dim3(1), dim3(64), 0, ctx->stream, (const int *)p->itemStart,
+                       slots, (const int *)p->starts, sh.nKeys, p->meta);
+    TCV_HIP(ctx, hipGetLastError());
+    return TCV_OK;
+}
+
+// ---- tiled launch dispatch ----------------------------------------------------
+
+namespace {
+
+constexpr int kNW = 4;   // waves per workgroup
+constexpr int kRB = 4;   // row groups per uniform range test
+
+template <int TW, int TH>
+struct TileCfg {
+    static constexpr int RPI = 64 / TW;
+    static constexpr int RG = TH / (kNW * RPI);   // gridder rows groups per wave
+    static constexpr int RD = TH / RPI;           // degridder row groups per wave
+};
+
+int persistent_blocks(tcv_ctx *ctx, const void *kernel, int bs)
+{
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, bs, 0) != hipSuccess || per < 1)
+        per = 1;
+    return std::max(1, ctx->cus) * per;
+}
+
+template <int TW, int TH>
+tcv_status grid_tiled_t(tcv_ctx *ctx, tcv_plan *p, const float2 *data, const float2 *C,
+                        float2 *grid)
+{
+    using T = TileCfg<TW, TH>;
+    auto kern = k_grid_tiled<TW, T::RG, kNW, (T::RG % kRB == 0 ? kRB : T::RG)>;
+    const int blocks = persistent_blocks(ctx, (const void *)kern, kNW * 64);
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, p->meta);
+    ProfScope ps(ctx, "grid_tiled");
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kNW * 64), 0, ctx->stream,
+                       (const int4 *)p->items, (const int4 *)p->entries, p->meta, p->sh.nTX,
+                       data, C, (unsigned)(p->cSize * 8), grid, p->gSize);
+    TCV_HIP(ctx, hipGetLastError());
+    return TCV_OK;
+}
+
+template <int TW, int TH>
+tcv_status degrid_tiled_t(tcv_ctx *ctx, tcv_plan *p, const float2 *grid, const float2 *C,
+                          float2 *out)
+{
+    using T = TileCfg<TW, TH>;
+    auto kern = k_degrid_tiled<TW, T::RD, kNW, (T::RD % kRB == 0 ? kRB : T::RD)>;
+    const int blocks = persistent_blocks(ctx, (const void *)kern, kNW * 64);
+    tcv_status st = launch_fill_u32(ctx, out, 0, (size_t)p->nVis * 2);
+    if (st != TCV_OK) return st;
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, p->meta);
+    ProfScope ps(ctx, "degrid_tiled");
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kNW * 64), 0, ctx->stream,
+                       (const int4 *)p->items, (const int4 *)p->entries, p->meta, p->sh.nTX,
+                       grid, p->gSize, C, (unsigned)(p->cSize * 8), out);
+    TCV_HIP(ctx, hipGetLastError());
+    return TCV_OK;
+}
+
+}  // namespace
+
+#define TCV_TILE_CONFIGS(X) \
+    X(64, 64) X(64, 32) X(32, 64) X(32, 32) X(16, 64) X(16, 32) X(8, 64) X(8, 32)
+
+bool tiled_config_supported(int tw, int th)
+{
+#define X(a, b) if (tw == a && th == b) return true;
+    TCV_TILE_CONFIGS(X)
+#undef X
+    return false;
+}
+
+tcv_status launch_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *data, const float2 *C,
+                             float2 *grid)
+{
+#define X(a, b) if (p->sh.tw == a && p->sh.th == b) return grid_tiled_t<a, b>(ctx, p, data, C, grid);
+    TCV_TILE_CONFIGS(X)
+#undef X
+    return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");
+}
+
+tcv_status launch_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *grid, const float2 *C,
+                               float2 *out)
+{
+#define X(a, b) if (p->sh.tw == a && p->sh.th == b) return degrid_tiled_t<a, b>(ctx, p, grid, C, out);
+    TCV_TILE_CONFIGS(X)
+#undef X
+    return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");
+}

@@ -1,0 +1,243 @@
+"""Device context, device arrays, geometry and plans over the C ABI."""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from ._lib import STATUS, Geometry, Tuning, lib
+
+P = ct.c_void_p
+
+
+class TcvError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        self.status = status
+        self.code = STATUS.get(status, str(status))
+        super().__init__(f"{self.code}: {message}")
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(P)
+
+
+class Context:
+    """One device, one stream (tcv_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = P()
+        st = lib().tcv_create(device, ct.byref(h))
+        if st:
+            raise TcvError(st, f"tcv_create(device={device}) failed")
+        self.h = h
+        self.device = device
+
+    # -- errors -------------------------------------------------------------
+    def check(self, st: int, what: str = ""):
+        if st:
+            msg = lib().tcv_last_error(self.h)
+            raise TcvError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tcv_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def cus(self) -> int:
+        return lib().tcv_device_cus(self.h)
+
+    def sync(self):
+        self.check(lib().tcv_synchronize(self.h), "synchronize")
+
+    # -- memory -------------------------------------------------------------
+    def empty(self, shape, dtype) -> "DeviceArray":
+        return DeviceArray(self, shape, dtype)
+
+    def zeros(self, shape, dtype) -> "DeviceArray":
+        a = DeviceArray(self, shape, dtype)
+        a.zero()
+        return a
+
+    def to_device(self, host) -> "DeviceArray":
+        host = np.ascontiguousarray(host)
+        a = DeviceArray(self, host.shape, host.dtype)
+        a.upload(host)
+        return a
+
+    # -- timing -------------------------------------------------------------
+    def timer_start(self):
+        self.check(lib().tcv_timer_start(self.h), "timer_start")
+
+    def timer_stop(self) -> float:
+        ms = ct.c_double()
+        self.check(lib().tcv_timer_stop(self.h, ct.byref(ms)), "timer_stop")
+        return ms.value
+
+    def set_profiling(self, on: bool):
+        self.check(lib().tcv_set_profiling(self.h, int(on)), "set_profiling")
+
+    def kernel_stats(self, family: str):
+        n = ct.c_long()
+        ms = ct.c_double()
+        self.check(lib().tcv_kernel_stats(self.h, family.encode(), ct.byref(n), ct.byref(ms)),
+                   "kernel_stats")
+        return n.value, ms.value
+
+    def reset_stats(self):
+        self.check(lib().tcv_reset_stats(self.h), "reset_stats")
+
+    def range_errors(self) -> int:
+        c = ct.c_longlong()
+        self.check(lib().tcv_range_errors(self.h, ct.byref(c)), "range_errors")
+        return c.value
+
+
+class DeviceArray:
+    """A device buffer owned by a Context (freed with the object)."""
+
+    def __init__(self, ctx: Context, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(np.atleast_1d(shape).tolist()) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        h = P()
+        ctx.check(lib().tcv_malloc(ctx.h, self.nbytes, ct.byref(h)), f"malloc {self.nbytes} B")
+        self.ptr = h
+
+    @property
+    def size(self):
+        return int(np.prod(self.shape, dtype=np.int64))
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        if host.nbytes != self.nbytes:
+            raise ValueError(f"upload of {host.nbytes} B into a {self.nbytes} B buffer")
+        self.ctx.check(lib().tcv_upload(self.ctx.h, self.ptr, _p(host), self.nbytes), "upload")
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        self.ctx.check(lib().tcv_download(self.ctx.h, _p(out), self.ptr, self.nbytes), "download")
+        return out
+
+    def zero(self):
+        self.ctx.check(lib().tcv_zero(self.ctx.h, self.ptr, self.nbytes), "zero")
+
+    def free(self):
+        if getattr(self, "ptr", None) and self.ctx.h:
+            lib().tcv_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Visibilities:
+    """Geometry of one job: device index arrays plus the per-plane widths.
+
+    Mirrors the reference's per-visibility offsets (attic `Sample` iu/iv/cOffset,
+    attic/tConvolveMPI/Benchmark.h:48-53; current iu/iv/wPlane/cOffset,
+    current/tConvolveMPI/Benchmark.h:96-103)."""
+
+    def __init__(self, ctx: Context, iu, iv, cOffset, sSize, gSize, cSize, wPlane=None):
+        self.ctx = ctx
+        self.iu = iu if isinstance(iu, DeviceArray) else ctx.to_device(np.asarray(iu, np.int32))
+        self.iv = iv if isinstance(iv, DeviceArray) else ctx.to_device(np.asarray(iv, np.int32))
+        self.cOffset = (cOffset if isinstance(cOffset, DeviceArray)
+                        else ctx.to_device(np.asarray(cOffset, np.int32)))
+        self.wPlane = None
+        if wPlane is not None:
+            self.wPlane = (wPlane if isinstance(wPlane, DeviceArray)
+                           else ctx.to_device(np.asarray(wPlane, np.int32)))
+        self.sSize = np.ascontiguousarray(np.atleast_1d(sSize), dtype=np.int32)
+        self.gSize = int(gSize)
+        self.cSize = int(cSize)
+        self.nVis = self.iu.size
+
+    def geometry(self) -> Geometry:
+        g = Geometry()
+        g.nVis = self.nVis
+        g.d_iu = self.iu.ptr
+        g.d_iv = self.iv.ptr
+        g.d_cOffset = self.cOffset.ptr
+        g.d_wPlane = self.wPlane.ptr if self.wPlane is not None else None
+        g.gSize = self.gSize
+        g.nPlanes = int(self.sSize.size)
+        g.sSize = self.sSize.ctypes.data_as(P)
+        g.cSize = self.cSize
+        return g
+
+
+def grid_plain(ctx: Context, vis: Visibilities, data: DeviceArray, C: DeviceArray,
+               grid: DeviceArray):
+    g = vis.geometry()
+    ctx.check(lib().tcv_grid_plain(ctx.h, ct.byref(g), data.ptr, C.ptr, grid.ptr), "grid_plain")
+
+
+def degrid_plain(ctx: Context, vis: Visibilities, grid: DeviceArray, C: DeviceArray,
+                 out: DeviceArray):
+    g = vis.geometry()
+    ctx.check(lib().tcv_degrid_plain(ctx.h, ct.byref(g), grid.ptr, C.ptr, out.ptr),
+              "degrid_plain")
+
+
+class Plan:
+    """Device-side binning of a job's visibilities by grid tile (tcv_plan)."""
+
+    def __init__(self, ctx: Context, vis: Visibilities, tile_w=0, tile_h=0, chunk=0,
+                 slice_keys=0, phases=0):
+        self.ctx = ctx
+        self.vis = vis
+        t = Tuning(tile_w, tile_h, chunk, slice_keys, phases)
+        g = vis.geometry()
+        h = P()
+        ctx.check(lib().tcv_plan_create(ctx.h, ct.byref(g), ct.byref(t), ct.byref(h)),
+                  "plan_create")
+        self.h = h
+
+    def build(self, vis: Visibilities | None = None):
+        vis = vis or self.vis
+        g = vis.geometry()
+        self.ctx.check(lib().tcv_plan_build(self.ctx.h, self.h, ct.byref(g)), "plan_build")
+
+    def info(self):
+        t = Tuning()
+        e, it, bad = ct.c_longlong(), ct.c_longlong(), ct.c_longlong()
+        self.ctx.check(lib().tcv_plan_info(self.ctx.h, self.h, ct.byref(t), ct.byref(e),
+                                           ct.byref(it), ct.byref(bad)), "plan_info")
+        return dict(tile_w=t.tile_w, tile_h=t.tile_h, chunk=t.chunk, slice_keys=t.slice_keys,
+                    phases=t.phases, entries=e.value, items=it.value, out_of_range=bad.value)
+
+    def grid(self, data: DeviceArray, C: DeviceArray, grid: DeviceArray):
+        self.ctx.check(lib().tcv_grid_tiled(self.ctx.h, self.h, data.ptr, C.ptr, grid.ptr),
+                       "grid_tiled")
+
+    def degrid(self, grid: DeviceArray, C: DeviceArray, out: DeviceArray):
+        self.ctx.check(lib().tcv_degrid_tiled(self.ctx.h, self.h, grid.ptr, C.ptr, out.ptr),
+                       "degrid_tiled")
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib().tcv_plan_destroy(self.ctx.h, self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,103 @@
+"""Device-resident gridding jobs: offsets on the device, then plain or tiled
+grid / degrid passes.  Everything here calls libtcv; nothing runs on the CPU
+except argument marshalling."""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from ._lib import lib
+from .device import Context, DeviceArray, Plan, TcvError, Visibilities, degrid_plain, grid_plain
+from .setup import Problem
+
+P = ct.c_void_p
+
+
+def init_offsets(ctx: Context, u: DeviceArray, v: DeviceArray, w: DeviceArray, freq,
+                 cellSize: float, wCellSize: float, wSize: int, gSize: int, overSample: int,
+                 sSize, cOffset0=None, want_wplane: bool = True, allow_out_of_range=False):
+    """Benchmark::initCOffset on the device (FP64, bit-exact).
+
+    attic/tConvolveMPI/Benchmark.cc:281-324 when cOffset0 is None, else
+    current/tConvolveMPI/Benchmark.cc:597-681.  Returns device arrays
+    (iu, iv, wPlane or None, cOffset), numPerPlane and the out-of-range count."""
+    freq = np.ascontiguousarray(freq, np.float64)
+    sSize = np.ascontiguousarray(np.atleast_1d(sSize), np.int32)
+    co0 = None if cOffset0 is None else np.ascontiguousarray(cOffset0, np.int32)
+    nSamples = u.size
+    n = nSamples * freq.size
+    iu = ctx.empty(n, np.int32)
+    iv = ctx.empty(n, np.int32)
+    wp = ctx.empty(n, np.int32) if want_wplane else None
+    co = ctx.empty(n, np.int32)
+    npp = np.zeros(wSize, np.int64)
+    bad = ct.c_longlong()
+    st = lib().tcv_init_offsets(ctx.h, u.ptr, v.ptr, w.ptr, nSamples, freq.ctypes.data_as(P),
+                                freq.size, cellSize, wCellSize, wSize, gSize, overSample,
+                                sSize.ctypes.data_as(P),
+                                None if co0 is None else co0.ctypes.data_as(P),
+                                iu.ptr, iv.ptr, None if wp is None else wp.ptr, co.ptr,
+                                npp.ctypes.data_as(P), ct.byref(bad))
+    if st and not (st == 2 and allow_out_of_range):
+        ctx.check(st, "init_offsets")
+    return iu, iv, wp, co, npp, bad.value
+
+
+class GridJob:
+    """One gridding/degridding job resident on the device.
+
+    Holds the visibility geometry (offsets computed on the device from u, v,
+    w), the C(w) table, the data, the grid and the degridded output."""
+
+    def __init__(self, ctx: Context, prob: Problem, tuning: dict | None = None,
+                 make_plan: bool = True):
+        self.ctx = ctx
+        self.prob = prob
+        self.d_u = ctx.to_device(prob.u)
+        self.d_v = ctx.to_device(prob.v)
+        self.d_w = ctx.to_device(prob.w)
+        self.iu, self.iv, self.wp, self.co, self.numPerPlane, self.nBad = init_offsets(
+            ctx, self.d_u, self.d_v, self.d_w, prob.freq, prob.cellSize, prob.wCellSize,
+            prob.wSize, prob.gSize, prob.overSample, prob.sSize, prob.cOffset0,
+            want_wplane=prob.cOffset0 is not None)
+        self.C = ctx.to_device(prob.C)
+        self.data = ctx.to_device(prob.data)
+        self.grid_d = ctx.zeros((prob.gSize, prob.gSize), np.complex64)
+        self.out = ctx.zeros(prob.nVis, np.complex64)
+        self.vis = Visibilities(ctx, self.iu, self.iv, self.co, prob.sSize, prob.gSize,
+                                prob.C.size, wPlane=self.wp)
+        self.plan = Plan(ctx, self.vis, **(tuning or {})) if make_plan else None
+
+    @property
+    def points(self) -> int:
+        s = self.prob.sSize.astype(np.int64)
+        if self.prob.cOffset0 is None:
+            return int(self.prob.nVis) * int(s[0]) ** 2
+        return int((self.numPerPlane * s * s).sum())
+
+    def zero_grid(self):
+        self.grid_d.zero()
+
+    def grid(self, method: str = "tiled"):
+        if method == "tiled":
+            self.plan.grid(self.data, self.C, self.grid_d)
+        elif method == "plain":
+            grid_plain(self.ctx, self.vis, self.data, self.C, self.grid_d)
+        else:
+            raise ValueError(method)
+
+    def degrid(self, method: str = "tiled"):
+        if method == "tiled":
+            self.plan.degrid(self.grid_d, self.C, self.out)
+        elif method == "plain":
+            degrid_plain(self.ctx, self.vis, self.grid_d, self.C, self.out)
+        else:
+            raise ValueError(method)
+
+    def offsets_host(self):
+        wp = self.wp.download() if self.wp is not None else None
+        return self.iu.download(), self.iv.download(), wp, self.co.download()
+
+
+__all__ = ["init_offsets", "GridJob", "TcvError"]

@@ -1,0 +1,178 @@
+"""Benchmark: W-projection gridding + degridding throughput on one MI355X.
+
+Workload (spectral scale): 1 048 576 samples x 8 channels, u, v, w uniform in
+[-6000, 6000] m, complex Gaussian data, 4096^2 complex64 grid, wSize 129,
+overSample 8, cellSize 15 wavelengths (support 64, 129 x 129 taps).
+
+One step = re-bin the visibilities on the device (plan build) + zero the grid
++ grid every visibility + degrid every visibility, all on device-resident
+inputs.  value = (grid points gridded + grid points degridded) / step time.
+
+Prints one JSON line.  Extra fields: the plain one-workgroup-per-visibility
+kernels timed on the same data, the roofline of the dominant kernel and the
+CPU baseline (the oracle's restatement of gridKernel/degridKernel on the host
+cores, on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "askap-benchmarks_amd"))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def alg_bytes(points, nVis, gSize, op):
+    """Algorithmic HBM bytes: one complex64 C tap per grid point, the grid read
+    (degrid) or read+written (grid) once, 20 B per visibility (iu, iv, cOffset,
+    data)."""
+    grid_bytes = (16 if op == "grid" else 8) * gSize * gSize
+    return 8 * points + grid_bytes + 20 * nVis
+
+
+def cpu_baseline(prob, iu, iv, co, seconds_target=20.0):
+    import oracle as O
+    cores = min(os.cpu_count() or 1, 16)
+    S = int(prob.sSize[0])
+    # ~1.5 ns per point per core for grid and again for degrid (single core)
+    per_thread = int(max(2000, seconds_target / cores * 0.33e9 / (S * S)))
+    per_thread = min(per_thread, iu.size)
+    tg, td = O.cpu_baseline(cores, per_thread, prob.data, iu, iv, co, prob.C, prob.gSize, S)
+    pts = cores * per_thread * S * S
+    rate = 2 * pts / (tg + td)
+    return {
+        "value": rate, "unit": "grid points/s", "cores": cores, "kind": "port",
+        "sample": (f"{cores} threads x {per_thread} visibilities of the same workload, each "
+                   f"thread gridding into its own {prob.gSize}^2 grid then degridding "
+                   f"(tConvolveMPI methodology, C shared read-only); grid {tg:.2f} s, "
+                   f"degrid {td:.2f} s"),
+        "grid_rate": pts / tg, "degrid_rate": pts / td,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nsamples", type=int, default=1 << 20)
+    ap.add_argument("--nchan", type=int, default=8)
+    ap.add_argument("--tile-w", type=int, default=0)
+    ap.add_argument("--tile-h", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--slice-keys", type=int, default=0)
+    ap.add_argument("--phases", type=int, default=0)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: the gridding path here runs on a single "
+              f"MI355X (requested --gpus {args.gpus}); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import tcv
+    from tcv import setup as S
+
+    t0 = time.time()
+    prob = S.config2(nSamples=args.nsamples, nChan=args.nchan)
+    log(f"problem built in {time.time() - t0:.1f} s: {prob.nVis} visibilities, "
+        f"C {prob.C.nbytes / 2**30:.2f} GiB")
+    ctx = tcv.Context(0)
+    tuning = dict(tile_w=args.tile_w, tile_h=args.tile_h, chunk=args.chunk,
+                  slice_keys=args.slice_keys, phases=args.phases)
+    job = tcv.GridJob(ctx, prob, tuning=tuning)
+    assert job.nBad == 0, f"{job.nBad} visibilities out of range"
+    points = job.points
+    info = job.plan.info()
+    log(f"plan: {info}")
+
+    def step():
+        job.plan.build()
+        job.zero_grid()
+        job.grid("tiled")
+        job.degrid("tiled")
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    ctx.sync()
+    w0 = time.perf_counter()
+    ctx.timer_start()
+    for _ in range(args.steps):
+        step()
+    ms_total = ctx.timer_stop()
+    wall = time.perf_counter() - w0
+    stats = {f: ctx.kernel_stats(f) for f in ("plan", "grid_tiled", "degrid_tiled")}
+    ctx.set_profiling(False)
+    ms_step = ms_total / args.steps
+    value = 2 * points / (ms_step * 1e-3)
+    log(f"tiled: {ms_step:.2f} ms/step, {value / 1e9:.1f} G points/s (wall {wall:.2f} s)")
+
+    # roofline of the dominant kernel
+    fams = {k: v for k, v in stats.items() if k != "plan" and v[0] > 0}
+    dom = max(fams, key=lambda k: fams[k][1])
+    n_l, ms_l = fams[dom]
+    avg_ms = ms_l / n_l
+    op = "grid" if dom.startswith("grid") else "degrid"
+    bytes_l = alg_bytes(points, prob.nVis, prob.gSize, op)
+    achieved = bytes_l / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "avg_launch_ms": avg_ms, "alg_bytes_per_launch": bytes_l}
+    per_kernel = {k: {"launches": v[0], "avg_ms": (v[1] / v[0]) if v[0] else None}
+                  for k, v in stats.items()}
+
+    result = {
+        "metric": "grid points/s (grid+degrid) batched vs plain, 1 GPU; p50/p99 job latency",
+        "value": value, "unit": "grid points/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 (complex64)",
+        "data": "synthetic (seeded: numpy PCG64 u,v,w uniform [-6000,6000] m, complex Gaussian data)",
+        "config": {"workload": "spectral: 1048576 samples x 8 channels, 4096^2 grid, wSize 129, "
+                               "overSample 8, support 64 (129x129 taps), tiled gridder+degridder",
+                   "nVis": prob.nVis, "gSize": prob.gSize, "sSize": int(prob.sSize[0]),
+                   "grid_points_per_pass": points, "tuning": info},
+        "roofline": roofline,
+        "kernels": per_kernel,
+    }
+
+    if not args.no_plain:
+        job.zero_grid()
+        ctx.sync()
+        ctx.timer_start()
+        job.grid("plain")
+        t_gp = ctx.timer_stop()
+        ctx.timer_start()
+        job.degrid("plain")
+        t_dp = ctx.timer_stop()
+        plain = 2 * points / ((t_gp + t_dp) * 1e-3)
+        result["plain"] = {"value": plain, "grid_ms": t_gp, "degrid_ms": t_dp,
+                           "speedup_tiled_vs_plain": value / plain}
+        log(f"plain: grid {t_gp:.1f} ms, degrid {t_dp:.1f} ms, {plain / 1e9:.1f} G points/s")
+
+    if not args.no_cpu:
+        import oracle as O
+        iu, iv, co = O.attic_initCOffset(prob.u, prob.v, prob.w, prob.freq, prob.cellSize,
+                                         prob.wCellSize, prob.wSize, prob.gSize, prob.support, 8)
+        cb = cpu_baseline(prob, iu, iv, co)
+        result["cpu_baseline"] = cb
+        log(f"cpu baseline: {cb['value'] / 1e6:.0f} M points/s on {cb['cores']} cores")
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

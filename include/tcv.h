@@ -1,0 +1,213 @@
+/*
+ * tcv.h -- C ABI of libtcv.so, the MI355X (gfx950) W-projection gridder /
+ * degridder that drops in for tConvolve's Benchmark hot path.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Arguments named d_* are device pointers
+ *    (allocated with tcv_malloc or any hipMalloc'd memory on the context's
+ *    device); everything else is host memory.
+ *  - Complex values are interleaved float pairs (re, im): the memory layout of
+ *    std::complex<float> (the reference's `Value`, attic/tConvolveMPI/Benchmark.h:36).
+ *  - Every call returns a tcv_status; tcv_last_error() gives the message.  The
+ *    library never calls exit() or abort().
+ *  - Work is queued on the context's stream and returns without waiting unless
+ *    stated otherwise; tcv_synchronize() waits.  One context must not be used
+ *    from two threads at once without external locking.
+ *  - Gridding ACCUMULATES into the grid (as gridKernel does); degridding
+ *    OVERWRITES the output visibilities (as degridKernel does).
+ */
+#ifndef TCV_H
+#define TCV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    TCV_OK = 0,
+    TCV_ERR_INVALID = 1,      /* bad argument */
+    TCV_ERR_RANGE = 2,        /* footprint / w-plane / C offset outside grid or table */
+    TCV_ERR_DEVICE = 3,       /* HIP runtime error */
+    TCV_ERR_NOMEM = 4,        /* device allocation failed */
+    TCV_ERR_BUDGET = 5,       /* admission refused: HBM budget exceeded */
+    TCV_ERR_UNSUPPORTED = 6,  /* shape outside what this build handles */
+    TCV_ERR_CANCELLED = 7     /* request cancelled or timed out */
+} tcv_status;
+
+typedef struct tcv_ctx tcv_ctx;
+
+/* ---- context ------------------------------------------------------------- */
+
+/* Opens `device` and creates the context's stream (default priority). */
+tcv_status tcv_create(int device, tcv_ctx **ctx);
+void tcv_destroy(tcv_ctx *ctx);
+const char *tcv_last_error(const tcv_ctx *ctx);
+const char *tcv_status_string(tcv_status s);
+/* Number of compute units of the context's device. */
+int tcv_device_cus(const tcv_ctx *ctx);
+tcv_status tcv_synchronize(tcv_ctx *ctx);
+
+/* ---- device memory -------------------------------------------------------- */
+
+tcv_status tcv_malloc(tcv_ctx *ctx, size_t bytes, void **d_ptr);
+tcv_status tcv_free(tcv_ctx *ctx, void *d_ptr);
+/* Blocking copies (they synchronise the context stream first). */
+tcv_status tcv_upload(tcv_ctx *ctx, void *d_dst, const void *src, size_t bytes);
+tcv_status tcv_download(tcv_ctx *ctx, void *dst, const void *d_src, size_t bytes);
+/* Zero fill by kernel (safe inside a captured graph). */
+tcv_status tcv_zero(tcv_ctx *ctx, void *d_ptr, size_t bytes);
+
+/* ---- timing (hipEvents on the context stream) ----------------------------- */
+
+tcv_status tcv_timer_start(tcv_ctx *ctx);
+/* Records the stop event, waits for it, returns the elapsed milliseconds. */
+tcv_status tcv_timer_stop(tcv_ctx *ctx, double *ms);
+/* When on, every gridding/degridding kernel launch is bracketed by events on
+ * the launch stream; tcv_kernel_stats() waits and reports per kernel family
+ * ("grid_plain", "degrid_plain", "grid_tiled", "degrid_tiled", "offsets",
+ * "plan") the launch count and summed device milliseconds since the last
+ * reset. */
+tcv_status tcv_set_profiling(tcv_ctx *ctx, int on);
+tcv_status tcv_kernel_stats(tcv_ctx *ctx, const char *family, long *launches,
+                            double *total_ms);
+tcv_status tcv_reset_stats(tcv_ctx *ctx);
+
+/* ---- host set-up: the Benchmark data generators ---------------------------- */
+
+/* Benchmark::randomInt -- attic/tConvolveMPI/Benchmark.cc:53-58 (64-bit LCG). */
+int tcv_random_int(uint64_t *next);
+
+/* Benchmark::init, u/v/w part -- attic/tConvolveMPI/Benchmark.cc:69-80. */
+void tcv_attic_uvw(int nSamples, int baseline, uint64_t *next, double *u,
+                   double *v, double *w);
+
+/* Benchmark::initC (attic: real C, uniform support) --
+ * attic/tConvolveMPI/Benchmark.cc:206-270.  Host computation (the table is
+ * set-up, bit-identical with the reference); C == NULL only returns support
+ * and wCellSize.  C holds sSize^2 * overSample^2 * wSize complex values,
+ * index i + sSize*(j + sSize*(osi + os*(osj + os*k))). */
+tcv_status tcv_attic_initC(double freq0, double cellSize, int wSize, int baseline,
+                           int overSample, int *support, double *wCellSize,
+                           float *C);
+
+/* Problem parameters of current/tConvolveMPI runType 0..4 --
+ * current/tConvolveMPI/Benchmark.cc:64-156. */
+tcv_status tcv_current_params(int runType, int *gSize, double *uvCellSize,
+                              int *wSize, int *support, int *overSample,
+                              double *wCellSize, float *baseline);
+
+/* ASKAP-layout uvw generator -- current/tConvolveMPI/Benchmark.cc:168-258.
+ * Returns the sample count; u/v/w == NULL only sizes. */
+int tcv_current_uvw(float baseline, uint64_t *next, double *u, double *v,
+                    double *w);
+
+/* Benchmark::initC (current: complex Fresnel C, per-plane sSize) --
+ * current/tConvolveMPI/Benchmark.cc:471-586.  sSize/cOffset0 get wSize
+ * entries (sSize[0] is an input when wSize == 1).  Returns the tap count in
+ * *nTaps; C == NULL only sizes. */
+tcv_status tcv_current_initC(double uvCellSize, int wSize, int support,
+                             int overSample, double wCellSize, int *sSize,
+                             int *cOffset0, float *C, long long *nTaps);
+
+/* ---- a6: index set-up on the device (FP64, bit-exact) ---------------------- */
+
+/* Benchmark::initCOffset -- attic/tConvolveMPI/Benchmark.cc:281-324 (when
+ * cOffset0 == NULL: uniform kernel width sSize[0], cOffset =
+ * s^2*(fracu + os*(fracv + os*woff))) and current/tConvolveMPI/Benchmark.cc:597-681
+ * (cOffset0 != NULL: per-plane widths, cOffset = s_w^2*(fracu + os*fracv) +
+ * cOffset0[w]; woff = 0 when wCellSize <= 0).  Visibility d = i*nChan + chan.
+ * freq, sSize, cOffset0 are host arrays (nChan, wSize, wSize entries).
+ * d_wPlane may be NULL; numPerPlane (host, wSize entries) may be NULL.
+ * Indices are computed in FP64 without contraction, bit-identical to the
+ * reference.  Blocking.  Returns TCV_ERR_RANGE (outputs still written) when
+ * any w-plane is outside [0, wSize) or a footprint leaves the gSize grid; the
+ * count is in *nOutOfRange when non-NULL. */
+tcv_status tcv_init_offsets(tcv_ctx *ctx, const double *d_u, const double *d_v,
+                            const double *d_w, int nSamples, const double *freq,
+                            int nChan, double cellSize, double wCellSize,
+                            int wSize, int gSize, int overSample,
+                            const int *sSize, const int *cOffset0, int *d_iu,
+                            int *d_iv, int *d_wPlane, int *d_cOffset,
+                            long long *numPerPlane, long long *nOutOfRange);
+
+/* ---- geometry of one gridding job ----------------------------------------- */
+
+typedef struct {
+    int nVis;                 /* visibilities (nSamples * nChan) */
+    const int *d_iu;          /* grid column of the footprint centre */
+    const int *d_iv;          /* grid row of the footprint centre */
+    const int *d_cOffset;     /* start of the visibility's C slice (complex units) */
+    const int *d_wPlane;      /* w-plane per visibility; NULL => uniform width */
+    int gSize;                /* grid is gSize x gSize, row-major, index iu + gSize*iv */
+    int nPlanes;              /* entries in sSize (1 when uniform) */
+    const int *sSize;         /* HOST: kernel width per plane (odd, 2*support+1) */
+    long long cSize;          /* complex taps in C (for bounds validation) */
+} tcv_geometry;
+
+/* ---- a7/a8 plain path: the one-job-per-launch kernels ---------------------- */
+
+/* gridKernel -- attic/tConvolveMPI/Benchmark.cc:125-155 (replaces
+ * gridKernelCuda, attic/tConvolveCuda/CudaGridder.h:32-40, and
+ * cuda_gridKernel, attic/tConvolveCuda/CudaGridKernel.h:33-37) with device
+ * pointers: grid[iu-s/2+u, iv-s/2+v] += data * C[cOffset + v*s + u] with FP32
+ * atomics, one workgroup per visibility.  Visibilities whose footprint leaves
+ * the grid or whose slice leaves C are skipped and counted (see
+ * tcv_range_errors). */
+tcv_status tcv_grid_plain(tcv_ctx *ctx, const tcv_geometry *g,
+                          const float *d_data, const float *d_C, float *d_grid);
+
+/* degridKernel -- attic/tConvolveMPI/Benchmark.cc:158-195 (replaces
+ * degridKernelCuda, attic/tConvolveCuda/CudaGridder.h:42-50): data =
+ * sum grid*C over the footprint, one wave64 per visibility with a cross-lane
+ * (DPP) reduction.  Skipped visibilities get 0. */
+tcv_status tcv_degrid_plain(tcv_ctx *ctx, const tcv_geometry *g,
+                            const float *d_grid, const float *d_C, float *d_data);
+
+/* Waits and returns (then clears) the number of visibilities skipped by the
+ * kernels above because their footprint or slice was out of range. */
+tcv_status tcv_range_errors(tcv_ctx *ctx, long long *count);
+
+/* ---- a7/a8 tiled path: visibilities binned by grid tile on the device ------- */
+
+typedef struct {
+    int tile_w;      /* 8, 16, 32 or 64 columns (0 = auto) */
+    int tile_h;      /* rows per tile (multiple of 64/tile_w; 0 = auto) */
+    int chunk;       /* max entries per work item (0 = auto) */
+    int slice_keys;  /* order entries in a tile by C-slice key, 0 = no ordering */
+    int phases;      /* >1: split each tile's entries into C-slice phases (0 = auto) */
+} tcv_tuning;
+
+typedef struct tcv_plan tcv_plan;
+
+/* Allocates a plan for jobs of this geometry class (device buffers sized from
+ * the geometry's upper bounds) and builds it from g's index arrays.  tuning
+ * may be NULL (auto).  Blocking. */
+tcv_status tcv_plan_create(tcv_ctx *ctx, const tcv_geometry *g,
+                           const tcv_tuning *tuning, tcv_plan **plan);
+/* Re-bins the visibilities of g (same nVis/gSize/sSize class as at creation;
+ * index arrays may differ).  Queued on the stream, no host sync. */
+tcv_status tcv_plan_build(tcv_ctx *ctx, tcv_plan *plan, const tcv_geometry *g);
+tcv_status tcv_plan_destroy(tcv_ctx *ctx, tcv_plan *plan);
+/* The tuning the plan resolved to, and after a build the number of entries
+ * (visibility x tile pairs) and work items (blocking). */
+tcv_status tcv_plan_info(tcv_ctx *ctx, tcv_plan *plan, tcv_tuning *tuning,
+                         long long *entries, long long *items,
+                         long long *outOfRange);
+
+/* Tiled gridKernel: each workgroup owns a grid tile in registers, walks the
+ * tile's binned visibilities, and writes the tile back once (exclusive tiles)
+ * or with FP32 atomics (tiles split over several work items). */
+tcv_status tcv_grid_tiled(tcv_ctx *ctx, tcv_plan *plan, const float *d_data,
+                          const float *d_C, float *d_grid);
+/* Tiled degridKernel: the tile is read once into registers; each visibility's
+ * partial sum over the tile is reduced with DPP and added to the output. */
+tcv_status tcv_degrid_tiled(tcv_ctx *ctx, tcv_plan *plan, const float *d_grid,
+                            const float *d_C, float *d_data);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -192,6 +192,24 @@ def _vis_args(iu, iv, cOffset, wPlane, sSize):
             _c(wPlane, np.int32), _c(sSize, np.int32))
 
 
+def _check_footprints(iu, iv, cOffset, wPlane, sSz, sUni, gSize, nTaps):
+    """The C loops, like the reference, do no bounds checks: refuse inputs whose
+    footprint (rows [iv, iv+s), columns [iu-s/2, iu-s/2+s)) leaves the grid or
+    whose slice leaves C, instead of writing past the arrays."""
+    if wPlane is not None:
+        if wPlane.size and (wPlane.min() < 0 or wPlane.max() >= sSz.size):
+            raise ValueError("w-plane index outside the sSize table")
+        s = sSz[wPlane].astype(np.int64)
+    else:
+        s = np.full(iu.size, int(sUni), np.int64)
+    u0 = iu.astype(np.int64) - s // 2
+    v0 = iv.astype(np.int64)
+    c0 = cOffset.astype(np.int64)
+    bad = (u0 < 0) | (v0 < 0) | (u0 + s > gSize) | (v0 + s > gSize) | (c0 < 0) | (c0 + s * s > nTaps)
+    if bad.any():
+        raise ValueError(f"{int(bad.sum())} visibilities have footprints outside the grid or C")
+
+
 def grid(data, iu, iv, cOffset, C, gSize, sSize, wPlane=None, out=None):
     """gridKernel: accumulates into `out` (complex64 gSize x gSize, created zero)."""
     data = _c(data, np.complex64)
@@ -201,6 +219,9 @@ def grid(data, iu, iv, cOffset, C, gSize, sSize, wPlane=None, out=None):
     sUni = 0 if wPlane is not None else int(sSize)
     if out is None:
         out = np.zeros((gSize, gSize), np.complex64)
+    if out.shape != (gSize, gSize) or out.dtype != np.complex64 or not out.flags.c_contiguous:
+        raise ValueError("out must be a C-contiguous complex64 gSize x gSize array")
+    _check_footprints(iu, iv, cOffset, wPlane, sSz, sUni, gSize, C.size)
     lib().orc_grid(data.size, _p(data), _p(iu), _p(iv), _p(cOffset), _p(wPlane),
                    _p(sSz), sUni, _p(C), _p(out), gSize)
     return out
@@ -214,6 +235,7 @@ def degrid(grid_, iu, iv, cOffset, C, sSize, wPlane=None):
                                              sSize if wPlane is not None else None)
     sUni = 0 if wPlane is not None else int(sSize)
     out = np.empty(iu.size, np.complex64)
+    _check_footprints(iu, iv, cOffset, wPlane, sSz, sUni, gSize, C.size)
     lib().orc_degrid(iu.size, _p(grid_), gSize, _p(iu), _p(iv), _p(cOffset),
                      _p(wPlane), _p(sSz), sUni, _p(C), _p(out))
     return out
@@ -226,6 +248,7 @@ def grid_f64(data, iu, iv, cOffset, C, gSize, sSize, wPlane=None):
                                              sSize if wPlane is not None else None)
     sUni = 0 if wPlane is not None else int(sSize)
     out = np.zeros((gSize, gSize), np.complex128)
+    _check_footprints(iu, iv, cOffset, wPlane, sSz, sUni, gSize, C.size)
     lib().orc_grid_f64(data.size, _p(data), _p(iu), _p(iv), _p(cOffset), _p(wPlane),
                        _p(sSz), sUni, _p(C), _p(out), gSize)
     return out
@@ -239,6 +262,7 @@ def degrid_f64(grid_, iu, iv, cOffset, C, sSize, wPlane=None):
                                              sSize if wPlane is not None else None)
     sUni = 0 if wPlane is not None else int(sSize)
     out = np.empty(iu.size, np.complex128)
+    _check_footprints(iu, iv, cOffset, wPlane, sSz, sUni, gSize, C.size)
     lib().orc_degrid_f64(iu.size, _p(grid_), gSize, _p(iu), _p(iv), _p(cOffset),
                          _p(wPlane), _p(sSz), sUni, _p(C), _p(out))
     return out
@@ -259,6 +283,7 @@ def cpu_baseline(nThreads, nVis, data, iu, iv, cOffset, C, gSize, sSize, wPlane=
     iu, iv, cOffset, wPlane, sSz = _vis_args(iu, iv, cOffset, wPlane,
                                              sSize if wPlane is not None else None)
     sUni = 0 if wPlane is not None else int(sSize)
+    _check_footprints(iu, iv, cOffset, wPlane, sSz, sUni, gSize, C.size)
     tg, td = D(), D()
     rc = lib().orc_cpu_baseline(nThreads, nVis, iu.size, _p(data), _p(iu), _p(iv),
                                 _p(cOffset), _p(wPlane), _p(sSz), sUni, _p(C), gSize,

@@ -1,0 +1,266 @@
+"""GPU parity: libtcv's kernels against the oracle (CPU restatement of the
+reference, itself bit-exact with the compiled reference) and the golden
+fixtures.  Bars: index set-up bit-exact; grid and degridded values
+max|d|/max|ref| <= 1e-5 on real and imaginary parts."""
+from __future__ import annotations
+
+import hashlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tcv
+from helpers import golden, rel_err, small_problem
+from tcv import setup as S
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+TILES = [(64, 64), (64, 32), (32, 64), (32, 32), (16, 64), (16, 32), (8, 64), (8, 32)]
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def vis_from_host(ctx, iu, iv, co, sSize, gSize, cSize, wPlane=None):
+    return tcv.Visibilities(ctx, iu, iv, co, sSize, gSize, cSize, wPlane=wPlane)
+
+
+def gpu_grid(ctx, vis, data, C, gSize, method, tuning=None, grid0=None):
+    d_data = ctx.to_device(data)
+    d_C = ctx.to_device(C)
+    g = ctx.to_device(grid0) if grid0 is not None else ctx.zeros((gSize, gSize), np.complex64)
+    if method == "plain":
+        tcv.grid_plain(ctx, vis, d_data, d_C, g)
+    else:
+        plan = tcv.Plan(ctx, vis, **(tuning or {}))
+        plan.grid(d_data, d_C, g)
+    return g.download()
+
+
+def gpu_degrid(ctx, vis, grid, C, method, tuning=None):
+    d_grid = ctx.to_device(grid)
+    d_C = ctx.to_device(C)
+    out = ctx.empty(vis.nVis, np.complex64)
+    if method == "plain":
+        tcv.degrid_plain(ctx, vis, d_grid, d_C, out)
+    else:
+        plan = tcv.Plan(ctx, vis, **(tuning or {}))
+        plan.degrid(d_grid, d_C, out)
+    return out.download()
+
+
+# ---- a6: index set-up, bit-exact ---------------------------------------------
+
+def test_offsets_config1_bitexact_vs_reference(ctx):
+    meta, arrs = golden("attic")
+    u, v, w = S.attic_uvw(meta["nSamples"], meta["baseline"])
+    freq = S.attic_freq(1)
+    sup, wcs, _ = S.attic_initC(freq[0], meta["cellSize"], meta["wSize"], meta["baseline"], 8,
+                                table=False)
+    assert sup == meta["support"] and wcs == meta["wCellSize"]
+    iu, iv, wp, co, npp, bad = tcv.init_offsets(
+        ctx, ctx.to_device(u), ctx.to_device(v), ctx.to_device(w), freq, meta["cellSize"], wcs,
+        meta["wSize"], meta["gSize"], 8, [2 * sup + 1], None)
+    assert bad == 0
+    assert sha(iu.download()) == meta["sha256"]["iu"]
+    assert sha(iv.download()) == meta["sha256"]["iv"]
+    assert sha(co.download()) == meta["sha256"]["cOffset"]
+    # w-plane histogram: only planes 8..24 are hit by [-1000, 1000] m
+    assert npp.sum() == meta["nSamples"] and npp[:8].sum() == 0 and npp[25:].sum() == 0
+
+
+@pytest.mark.parametrize("rt", [0, 1, 2, 3, 4])
+def test_offsets_current_bitexact_vs_reference(ctx, rt):
+    meta, arrs = golden(f"current_rt{rt}")
+    prob = S.current_problem(rt)
+    assert sha(prob.C) == meta["sha256"]["C"]
+    iu, iv, wp, co, npp, bad = tcv.init_offsets(
+        ctx, ctx.to_device(prob.u), ctx.to_device(prob.v), ctx.to_device(prob.w), prob.freq,
+        prob.cellSize, prob.wCellSize, prob.wSize, prob.gSize, prob.overSample, prob.sSize,
+        prob.cOffset0)
+    assert sha(iu.download()) == meta["sha256"]["iu"]
+    assert sha(iv.download()) == meta["sha256"]["iv"]
+    assert sha(wp.download()) == meta["sha256"]["wPlane"]
+    assert sha(co.download()) == meta["sha256"]["cOffset"]
+    assert np.array_equal(npp, arrs["numPerPlane"])
+
+
+def test_offsets_spectral_vs_oracle(ctx):
+    prob = S.config2(nSamples=1 << 16)
+    iu, iv, wp, co, npp, bad = tcv.init_offsets(
+        ctx, ctx.to_device(prob.u), ctx.to_device(prob.v), ctx.to_device(prob.w), prob.freq,
+        prob.cellSize, prob.wCellSize, prob.wSize, prob.gSize, 8, prob.sSize, None)
+    ref = O.attic_initCOffset(prob.u, prob.v, prob.w, prob.freq, prob.cellSize, prob.wCellSize,
+                              prob.wSize, prob.gSize, prob.support, 8)
+    assert bad == 0
+    for a, b in zip((iu, iv, co), ref):
+        assert np.array_equal(a.download(), b)
+
+
+# ---- a7/a8 on small seeded problems --------------------------------------------
+
+@pytest.mark.parametrize("method", ["plain", "tiled"])
+def test_single_visibility_patch_is_C_slice(ctx, method):
+    rng = np.random.default_rng(5)
+    Sz, gSize = 129, 512
+    iu, iv, co, data, C = small_problem(5, 1, gSize, Sz)
+    data[:] = 1.0 + 0.0j
+    vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
+    g = gpu_grid(ctx, vis, data, C, gSize, method)
+    u0, v0 = iu[0] - Sz // 2, iv[0]
+    patch = g[v0:v0 + Sz, u0:u0 + Sz]
+    assert np.array_equal(patch, C[co[0]:co[0] + Sz * Sz].reshape(Sz, Sz))
+    g[v0:v0 + Sz, u0:u0 + Sz] = 0
+    assert not g.any()
+    del rng
+
+
+@pytest.mark.parametrize("Sz", [129, 65, 33, 7])
+@pytest.mark.parametrize("method", ["plain", "tiled"])
+def test_grid_degrid_small_vs_oracle(ctx, method, Sz):
+    gSize = 384
+    iu, iv, co, data, C = small_problem(11 + Sz, 3000, gSize, Sz, complex_=True)
+    vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
+    g = gpu_grid(ctx, vis, data, C, gSize, method)
+    gref = O.grid(data, iu, iv, co, C, gSize, Sz)
+    assert rel_err(g, gref) <= TOL
+    g64 = O.grid_f64(data, iu, iv, co, C, gSize, Sz)
+    assert rel_err(g, g64) <= TOL
+    # degrid the oracle's grid
+    out = gpu_degrid(ctx, vis, gref, C, method)
+    oref = O.degrid(gref, iu, iv, co, C, Sz)
+    assert rel_err(out, oref) <= TOL
+
+
+@pytest.mark.parametrize("tw,th", TILES)
+def test_tiled_shapes_vs_oracle(ctx, tw, th):
+    Sz, gSize = 93, 300
+    iu, iv, co, data, C = small_problem(100 + tw + th, 1500, gSize, Sz, cluster=60)
+    vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
+    gref = O.grid(data, iu, iv, co, C, gSize, Sz)
+    for tuning in (dict(tile_w=tw, tile_h=th),
+                   dict(tile_w=tw, tile_h=th, chunk=64, slice_keys=256, phases=4)):
+        g = gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning)
+        assert rel_err(g, gref) <= TOL, tuning
+        out = gpu_degrid(ctx, vis, gref, C, "tiled", tuning)
+        assert rel_err(out, O.degrid(gref, iu, iv, co, C, Sz)) <= TOL, tuning
+
+
+def test_grid_accumulates_into_existing_grid(ctx):
+    Sz, gSize = 65, 256
+    iu, iv, co, data, C = small_problem(21, 500, gSize, Sz)
+    rng = np.random.default_rng(3)
+    g0 = (rng.standard_normal((gSize, gSize)) + 1j * rng.standard_normal((gSize, gSize))).astype(np.complex64)
+    vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
+    ref = O.grid(data, iu, iv, co, C, gSize, Sz, out=g0.copy())
+    for method in ("plain", "tiled"):
+        assert rel_err(gpu_grid(ctx, vis, data, C, gSize, method, grid0=g0), ref) <= TOL
+
+
+def test_adjointness_fp64(ctx):
+    """<G, grid(d)> == <degrid(G), d> (bilinear, no conjugation), in FP64."""
+    Sz, gSize = 129, 400
+    iu, iv, co, data, C = small_problem(31, 800, gSize, Sz)
+    vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
+    rng = np.random.default_rng(4)
+    G = (rng.standard_normal((gSize, gSize)) + 1j * rng.standard_normal((gSize, gSize))).astype(np.complex64)
+    for method in ("plain", "tiled"):
+        gd = gpu_grid(ctx, vis, data, C, gSize, method).astype(np.complex128)
+        dg = gpu_degrid(ctx, vis, G, C, method).astype(np.complex128)
+        lhs = np.sum(G.astype(np.complex128) * gd)
+        rhs = np.sum(dg * data.astype(np.complex128))
+        assert abs(lhs - rhs) <= 1e-5 * max(abs(lhs), 1.0)
+
+
+def test_out_of_range_visibilities_are_skipped(ctx):
+    Sz, gSize = 33, 128
+    iu, iv, co, data, C = small_problem(41, 200, gSize, Sz)
+    iu[:5] = 3            # footprint crosses the left edge
+    iv[5:10] = gSize - 2  # crosses the bottom edge
+    co[10:12] = C.size    # slice past the table
+    good = np.ones(iu.size, bool)
+    good[:12] = False
+    vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
+    ref = O.grid(data[good], iu[good], iv[good], co[good], C, gSize, Sz)
+    g = gpu_grid(ctx, vis, data, C, gSize, "plain")
+    assert ctx.range_errors() == 12
+    assert rel_err(g, ref) <= TOL
+    plan = tcv.Plan(ctx, vis)
+    assert plan.info()["out_of_range"] == 12
+    g2 = ctx.zeros((gSize, gSize), np.complex64)
+    plan.grid(ctx.to_device(data), ctx.to_device(C), g2)
+    assert rel_err(g2.download(), ref) <= TOL
+    out = ctx.empty(iu.size, np.complex64)
+    plan.degrid(ctx.to_device(ref), ctx.to_device(C), out)
+    o = out.download()
+    assert not o[~good].any()
+    assert rel_err(o[good], O.degrid(ref, iu[good], iv[good], co[good], C, Sz)) <= TOL
+
+
+# ---- the drop-in Benchmark surfaces ---------------------------------------------
+
+@pytest.mark.parametrize("method", ["tiled", "plain"])
+def test_attic_benchmark_config1_vs_reference(ctx, method):
+    """Full config-1 workload through the drop-in Benchmark; checked against the
+    compiled reference's fixture pixels and the oracle's full grid/degrid."""
+    meta, arrs = golden("attic")
+    b = tcv.Benchmark(ctx, method=method)
+    b.init()
+    assert b.getSupport() == 64 and b.nOutOfRange == 0
+    iu, iv, co = b.samples_host()
+    assert sha(co) == meta["sha256"]["cOffset"]
+    b.runGrid()
+    g = b.grid_host()
+    gv = g.reshape(-1)[arrs["grid_pix_idx"]]
+    assert rel_err(gv, arrs["grid_pix_val"]) <= TOL
+    C = b.C.download()
+    gref = O.grid(np.ones(iu.size, np.complex64), iu, iv, co, C, 4096, 129)
+    assert rel_err(g, gref) <= TOL
+    # degrid the reference's own grid so both sides see identical input
+    b.grid.upload(gref)
+    b.runDegrid()
+    out = b.outdata_host()
+    assert rel_err(out[:256], arrs["outdata_head"]) <= TOL
+    assert rel_err(out, O.degrid(gref, iu, iv, co, C, 129)) <= TOL
+
+
+@pytest.mark.parametrize("rt", [1, 3, 2])
+def test_current_benchmark_vs_oracle(ctx, rt):
+    """Variable per-plane widths and complex Fresnel C (runType 1), small
+    kernels with overSample 128 (runType 3), nearest neighbour (runType 2)."""
+    meta, arrs = golden(f"current_rt{rt}")
+    b = tcv.CurrentBenchmark(ctx)
+    b.setRunType(rt)
+    b.init()
+    assert b.nPixelsGridded() == meta["nPixelsGridded"]
+    b.runGrid()
+    g = b.grid_host()
+    gv = g.reshape(-1)[arrs["grid_pix_idx"]]
+    assert rel_err(gv, arrs["grid_pix_val"]) <= TOL
+    b.grid1.upload(np.zeros_like(g))
+    prob_iu, prob_iv = b.samples.iu.download(), b.samples.iv.download()
+    wp, co = b.samples.wPlane.download(), b.samples.cOffset.download()
+    C = b.C.download()
+    gref = O.grid(np.ones(prob_iu.size, np.complex64), prob_iu, prob_iv, co, C, b.gSize, b.sSize,
+                  wPlane=wp)
+    assert rel_err(g, gref) <= TOL
+    b.grid1.upload(gref)
+    b.runDegrid()
+    out = b.outdata_host()
+    oref = O.degrid(gref, prob_iu, prob_iv, co, C, b.sSize, wPlane=wp)
+    assert rel_err(out, oref) <= TOL
+    assert rel_err(out[:256], arrs["outdata_head"]) <= TOL
+
+
+def test_profiling_counts_launches(ctx):
+    Sz, gSize = 33, 128
+    iu, iv, co, data, C = small_problem(51, 100, gSize, Sz)
+    vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    gpu_grid(ctx, vis, data, C, gSize, "tiled")
+    n, ms = ctx.kernel_stats("grid_tiled")
+    ctx.set_profiling(False)
+    assert n == 1 and ms > 0

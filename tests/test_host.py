@@ -1,0 +1,111 @@
+"""Host side of the product (no GPU needed): libtcv loads, exports every
+symbol include/tcv.h declares, and its set-up code is bit-identical to the
+oracle and the compiled reference's fixtures."""
+from __future__ import annotations
+
+import ctypes as ct
+import hashlib
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tcv
+from helpers import golden
+from tcv import setup as S
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def declared_functions():
+    hdr = open(os.path.join(REPO, "include", "tcv.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    return sorted(set(re.findall(r"\b(tcv_[A-Za-z0-9_]+)\s*\(", hdr)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 30
+    L = ct.CDLL(tcv.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    nm = subprocess.run(["nm", "-D", "--defined-only", tcv.LIB_PATH], capture_output=True,
+                        text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (tcv_[A-Za-z0-9_]+)", nm))
+    assert set(names) <= exported
+    # the Python binding declares exactly the header's functions
+    assert set(tcv.exported_symbols()) == set(names)
+
+
+def test_status_strings():
+    L = tcv.lib()
+    assert L.tcv_status_string(0) == b"ok"
+    assert L.tcv_status_string(2) == b"index out of range"
+    assert L.tcv_status_string(99) == b"unknown status"
+
+
+def test_create_fails_cleanly_without_gpu():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    h = ct.c_void_p()
+    st = tcv.lib().tcv_create(0, ct.byref(h))
+    assert st != 0 and not h.value
+
+
+def test_lcg_and_attic_uvw_match_oracle_and_reference():
+    meta, arrs = golden("attic")
+    lcg = S.LCG()
+    assert [lcg.randomInt() for _ in range(64)] == arrs["lcg_draws"].tolist()
+    u, v, w = S.attic_uvw(160000, 2000)
+    assert sha(u) == meta["sha256"]["u"] and sha(v) == meta["sha256"]["v"]
+    assert sha(w) == meta["sha256"]["w"]
+
+
+def test_attic_initC_matches_reference():
+    meta, _ = golden("attic")
+    sup, wcs, C = S.attic_initC(S.attic_freq(1)[0], 5.0, 33, 2000, 8)
+    assert sup == 64 and wcs == meta["wCellSize"]
+    assert sha(C) == meta["sha256"]["C"]
+
+
+@pytest.mark.parametrize("rt", [0, 1, 2, 3, 4])
+def test_current_setup_matches_reference(rt):
+    meta, arrs = golden(f"current_rt{rt}")
+    p = S.current_params(rt)
+    for k in ("gSize", "uvCellSize", "wSize", "support", "overSample", "wCellSize"):
+        assert p[k] == meta[k]
+    u, v, w = S.current_uvw(p["baseline"])
+    assert sha(u) == meta["sha256"]["u"] and sha(w) == meta["sha256"]["w"]
+    sS, c0, C = S.current_initC(p["uvCellSize"], p["wSize"], p["support"], p["overSample"],
+                                p["wCellSize"])
+    assert sha(C) == meta["sha256"]["C"]
+    assert np.array_equal(sS, arrs["sSize"]) and np.array_equal(c0, arrs["cOffset0"])
+
+
+def test_current_wavenumber_and_required_rate():
+    assert S.current_wavenumber(1)[0] == 1.0e9 / 2.998e8
+    cont, spec = tcv.CurrentBenchmark.requiredRate()
+    # current/tConvolveMPI/Benchmark.cc:702-740: 373.2 / 639.6 Mpix/s per process
+    assert cont / 1e6 == pytest.approx(373.2, abs=0.05)
+    assert spec / 1e6 == pytest.approx(639.6, abs=0.05)
+
+
+def test_spectral_workload_fits_the_grid():
+    """Every channel of the spectral workload lands inside the grid and the
+    w-plane range (checked with the oracle's index set-up on a sample)."""
+    prob = S.config2(nSamples=1 << 15)
+    iu, iv, co = O.attic_initCOffset(prob.u, prob.v, prob.w, prob.freq, prob.cellSize,
+                                     prob.wCellSize, prob.wSize, prob.gSize, prob.support, 8)
+    s = int(prob.sSize[0])
+    assert prob.support == 64 and s == 129
+    assert (iu - 64 >= 0).all() and (iu + 64 < prob.gSize).all()
+    assert (iv >= 0).all() and (iv + s <= prob.gSize).all()
+    assert (co >= 0).all() and (co.astype(np.int64) + s * s <= prob.C.size).all()
+    assert abs(np.abs(prob.data) ** 2).mean() == pytest.approx(1.0, rel=0.02)

@@ -1,0 +1,276 @@
+"""Pins for the oracle (CPU, no GPU).
+
+Three kinds of evidence, chosen so a plausible mistake (dropped term, wrong
+sign or index, transposed operand) fails at least one:
+  * golden fixtures written from the compiled reference (tests/golden/),
+  * closed forms and invariants computed independently here,
+  * brute force on tiny inputs in pure Python.
+"""
+from __future__ import annotations
+
+import hashlib
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import golden, rel_err, small_problem
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+# ---- a1 LCG --------------------------------------------------------------------
+
+def test_lcg_closed_form_and_reference_draws():
+    meta, arrs = golden("attic")
+    lcg = O.LCG()
+    mine = [lcg.next() for _ in range(64)]
+    # closed form: 64-bit state, (uint32)(state / 65536) % (2^31 - 1)
+    state, expect = 1, []
+    for _ in range(64):
+        state = (state * 1103515245 + 12345) % (1 << 64)
+        expect.append(((state >> 16) & 0xFFFFFFFF) % 2147483647)
+    assert mine == expect
+    assert mine == arrs["lcg_draws"].tolist()
+    assert mine[:6] == [16838, 1507104382, 373696386, 1701364844, 1461254476, 1173460475]
+
+
+# ---- a2/a4 attic set-up ------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def attic():
+    meta, arrs = golden("attic")
+    u, v, w = O.attic_uvw(160000, 2000)
+    freq = O.attic_freq(1)
+    sup, wcs, C = O.attic_initC(freq[0], 5.0, 33, 2000)
+    iu, iv, co = O.attic_initCOffset(u, v, w, freq, 5.0, wcs, 33, 4096, sup)
+    return dict(meta=meta, arrs=arrs, u=u, v=v, w=w, freq=freq, sup=sup, wcs=wcs, C=C,
+                iu=iu, iv=iv, co=co)
+
+
+def test_attic_scalars(attic):
+    f0 = 1.4e9 / 2.998e8
+    assert attic["freq"][0] == f0
+    # printed by the reference: Support = 64, W cellsize = 566.034, 268 MB
+    assert attic["sup"] == int(1.5 * math.sqrt(2000 * 5.0 * f0) / 5.0) == 64
+    assert attic["wcs"] == pytest.approx(566.034, abs=5e-4)
+    assert attic["wcs"] == attic["meta"]["wCellSize"]
+    assert attic["C"].nbytes // (1024 * 1024) == 268
+    assert attic["C"].size == 129 * 129 * 8 * 8 * 33
+
+
+def test_attic_uvw_vs_reference(attic):
+    m, a = attic["meta"], attic["arrs"]
+    for k in ("u", "v", "w"):
+        assert sha(attic[k]) == m["sha256"][k]
+    assert attic["u"].min() >= -1000 and attic["u"].max() <= 1000
+
+
+def test_attic_C_closed_form_and_reference(attic):
+    C = attic["C"]
+    assert sha(C) == attic["meta"]["sha256"]["C"]
+    assert not C.imag.any()
+    # closed form before normalisation, recomputed here; the normalisation
+    # is one common float factor
+    f0, wcs, s, c = attic["freq"][0], attic["wcs"], 129, 64
+    raw = []
+    idx = []
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        k, osj, osi, j, i = (int(rng.integers(0, n)) for n in (33, 8, 8, s, s))
+        w = float(k - 33 // 2)
+        r2 = (j - c + osj / 8) ** 2 + (i - c + osi / 8) ** 2
+        val = math.exp(-r2) if w == 0 else math.cos(r2 / (w * math.sqrt(abs(w) * wcs * f0) / 5.0))
+        raw.append(np.float32(val))
+        idx.append(i + s * (j + s * (osi + 8 * (osj + 8 * k))))
+    raw = np.array(raw, np.float32)
+    # the w = 0 plane's centre tap is exp(0) = 1, so it holds the common
+    # normalisation factor exactly; every tap is float(raw) * that factor
+    centre = 64 + s * (64 + s * (0 + 8 * (0 + 8 * 16)))
+    scale = C.real[centre]
+    assert np.array_equal(C.real[idx], raw * scale)
+    # the factor is 2112 / (float running sum of |C|): that sum lies between
+    # 2^24 (where adding |C| <= 1 stalls) and the exact sum
+    sumC = np.float32(33 * 64) / scale
+    assert 2.0 ** 23 < sumC < 2.0 ** 25
+
+
+def test_attic_offsets_closed_form_and_reference(attic):
+    m = attic["meta"]
+    for k in ("iu", "iv"):
+        assert sha(attic[k]) == m["sha256"][k]
+    assert sha(attic["co"]) == m["sha256"]["cOffset"]
+    # expected first samples (derived by hand from the closed form)
+    assert attic["iu"][:3].tolist() == [1114, 2593, 1998]
+    assert attic["iv"][:3].tolist() == [2424, 2385, 2638]
+    assert attic["co"][:3].tolist() == [12647160, 17156871, 24595398]
+    # closed form in Python floats for a sample of visibilities
+    f0, wcs = attic["freq"][0], attic["wcs"]
+    for i in range(0, 160000, 997):
+        us = f0 * attic["u"][i] / 5.0
+        vs = f0 * attic["v"][i] / 5.0
+        iu, iv = math.floor(us), math.floor(vs)
+        fu, fv = int(8 * (us - iu)), int(8 * (vs - iv))
+        woff = 16 + int(f0 * attic["w"][i] / wcs)
+        assert attic["iu"][i] == iu + 2048 and attic["iv"][i] == iv + 2048
+        assert attic["co"][i] == 129 * 129 * (fu + 8 * (fv + 8 * woff))
+        assert 0 <= fu < 8 and 0 <= fv < 8 and 8 <= woff <= 24
+
+
+@pytest.mark.slow
+def test_attic_grid_degrid_vs_reference(attic):
+    """Full config-1 grid and degrid, bit-identical to the compiled reference."""
+    data = np.ones(160000, np.complex64)
+    g = O.grid(data, attic["iu"], attic["iv"], attic["co"], attic["C"], 4096, 129)
+    assert sha(g) == attic["meta"]["sha256"]["grid"]
+    out = O.degrid(g, attic["iu"], attic["iv"], attic["co"], attic["C"], 129)
+    assert sha(out) == attic["meta"]["sha256"]["outdata"]
+    # conservation: sum of the grid = sum over visibilities of the slice sums
+    slice_sums = np.array([attic["C"][c:c + 129 * 129].astype(np.complex128).sum()
+                           for c in attic["co"][:2000]])
+    g2 = O.grid(data[:2000], attic["iu"][:2000], attic["iv"][:2000], attic["co"][:2000],
+                attic["C"], 4096, 129)
+    assert abs(g2.astype(np.complex128).sum() - slice_sums.sum()) <= 1e-4 * np.abs(slice_sums).sum()
+
+
+# ---- a5/a6 current generation -----------------------------------------------------
+
+@pytest.mark.parametrize("rt", [0, 1, 2, 3, 4])
+def test_current_setup_vs_reference(rt):
+    meta, arrs = golden(f"current_rt{rt}")
+    p = O.current_params(rt)
+    for k in ("gSize", "uvCellSize", "wSize", "support", "overSample", "wCellSize"):
+        assert p[k] == meta[k], k
+    u, v, w = O.current_uvw(p["baseline"])
+    assert u.size == meta["nSamples"]
+    for k, a in (("u", u), ("v", v), ("w", w)):
+        assert sha(a) == meta["sha256"][k]
+    sS, c0, C = O.current_initC(p["uvCellSize"], p["wSize"], p["support"], p["overSample"],
+                                p["wCellSize"])
+    assert sha(C) == meta["sha256"]["C"]
+    assert np.array_equal(sS, arrs["sSize"]) and np.array_equal(c0, arrs["cOffset0"])
+    # per-plane normalisation: sum |C| = overSample^2 on every plane (double sum of floats)
+    os2 = p["overSample"] ** 2
+    for k in range(0, p["wSize"], max(1, p["wSize"] // 7)):
+        n = int(sS[k]) ** 2 * os2
+        tot = np.abs(C[c0[k]:c0[k] + n].astype(np.complex128)).sum()
+        assert tot == pytest.approx(os2, rel=1e-5)
+    # widths odd, >= 7 except the single-plane cases, centre plane narrowest
+    assert (sS % 2 == 1).all()
+    wn = np.array([1.0e9 / 2.998e8])
+    iu, iv, wp, co, npp = O.current_initCOffset(u, v, w, wn, p["uvCellSize"], p["wCellSize"],
+                                                p["wSize"], p["gSize"], p["overSample"], sS, c0)
+    for k, a in (("iu", iu), ("iv", iv), ("wPlane", wp), ("cOffset", co)):
+        assert sha(a) == meta["sha256"][k]
+    assert np.array_equal(npp, arrs["numPerPlane"])
+    assert int((npp * sS.astype(np.int64) ** 2).sum()) == meta["nPixelsGridded"]
+
+
+@pytest.mark.parametrize("rt", [1, 3])
+def test_current_grid_vs_reference(rt):
+    meta, arrs = golden(f"current_rt{rt}")
+    p = O.current_params(rt)
+    u, v, w = O.current_uvw(p["baseline"])
+    sS, c0, C = O.current_initC(p["uvCellSize"], p["wSize"], p["support"], p["overSample"],
+                                p["wCellSize"])
+    iu, iv, wp, co, npp = O.current_initCOffset(u, v, w, np.array([1.0e9 / 2.998e8]),
+                                                p["uvCellSize"], p["wCellSize"], p["wSize"],
+                                                p["gSize"], p["overSample"], sS, c0)
+    g = O.grid(np.ones(iu.size, np.complex64), iu, iv, co, C, p["gSize"], sS, wPlane=wp)
+    assert sha(g) == meta["sha256"]["grid"]
+    out = O.degrid(g, iu, iv, co, C, sS, wPlane=wp)
+    assert sha(out) == meta["sha256"]["outdata"]
+
+
+# ---- a7/a8 kernels: invariants and brute force ------------------------------------
+
+def test_single_visibility_patch_equals_slice():
+    iu, iv, co, data, C = small_problem(1, 1, 300, 129)
+    data[:] = 1
+    g = O.grid(data, iu, iv, co, C, 300, 129)
+    u0, v0 = iu[0] - 64, iv[0]
+    assert np.array_equal(g[v0:v0 + 129, u0:u0 + 129], C[co[0]:co[0] + 129 * 129].reshape(129, 129))
+    g[v0:v0 + 129, u0:u0 + 129] = 0
+    assert not g.any()
+
+
+def test_brute_force_tiny_complex():
+    """Pure-Python loops with complex arithmetic on a tiny problem."""
+    S, gSize = 5, 16
+    iu, iv, co, data, C = small_problem(2, 7, gSize, S, os_=2, nPlanes=2)
+    ref = [[0j] * gSize for _ in range(gSize)]
+    for d in range(7):
+        for y in range(S):
+            for x in range(S):
+                ref[iv[d] + y][iu[d] - 2 + x] += complex(data[d]) * complex(C[co[d] + y * S + x])
+    ref = np.array(ref)
+    g = O.grid(data, iu, iv, co, C, gSize, S)
+    assert np.abs(g - ref).max() <= 1e-5 * np.abs(ref).max()
+    # degrid brute force (no conjugation)
+    G = (np.arange(gSize * gSize).reshape(gSize, gSize) * (1 - 0.5j) / 100).astype(np.complex64)
+    out = O.degrid(G, iu, iv, co, C, S)
+    for d in range(7):
+        s = sum(complex(G[iv[d] + y, iu[d] - 2 + x]) * complex(C[co[d] + y * S + x])
+                for y in range(S) for x in range(S))
+        assert abs(out[d] - s) <= 1e-5 * max(abs(s), 1)
+
+
+def test_linearity_and_adjointness():
+    S, gSize = 33, 128
+    iu, iv, co, data, C = small_problem(3, 300, gSize, S)
+    a, b = np.complex64(1.5 - 0.25j), np.complex64(-0.75 + 2j)
+    d2 = np.roll(data, 7).astype(np.complex128)
+    g_lin = O.grid_f64(a * data.astype(np.complex128) + b * d2, iu, iv, co, C, gSize, S)
+    g_sep = a * O.grid_f64(data, iu, iv, co, C, gSize, S) + b * O.grid_f64(d2, iu, iv, co, C, gSize, S)
+    assert np.abs(g_lin - g_sep).max() <= 1e-9 * np.abs(g_sep).max()
+    rng = np.random.default_rng(9)
+    G = rng.standard_normal((gSize, gSize)) + 1j * rng.standard_normal((gSize, gSize))
+    lhs = np.sum(G * O.grid_f64(data, iu, iv, co, C, gSize, S))
+    rhs = np.sum(O.degrid_f64(G, iu, iv, co, C, S) * data)
+    assert abs(lhs - rhs) <= 1e-9 * abs(lhs)
+
+
+def test_degrid_of_ones_is_slice_sum():
+    """The attic CUDA check (attic/tConvolveCuda/tConvolveCuda.cc:395-420),
+    extended to the imaginary part with complex C."""
+    S, gSize = 65, 200
+    iu, iv, co, data, C = small_problem(4, 100, gSize, S)
+    ones = np.ones((gSize, gSize), np.complex64)
+    out = O.degrid(ones, iu, iv, co, C, S)
+    ref = np.array([C[c:c + S * S].astype(np.complex128).sum() for c in co])
+    assert rel_err(out, ref) <= 1e-5
+
+
+def test_fp32_oracle_close_to_fp64():
+    S, gSize = 129, 400
+    iu, iv, co, data, C = small_problem(5, 2000, gSize, S, cluster=40)
+    g32 = O.grid(data, iu, iv, co, C, gSize, S)
+    g64 = O.grid_f64(data, iu, iv, co, C, gSize, S)
+    assert rel_err(g32, g64) <= 1e-5
+    o32 = O.degrid(g32, iu, iv, co, C, S)
+    o64 = O.degrid_f64(g64, iu, iv, co, C, S)
+    assert rel_err(o32, o64) <= 1e-5
+
+
+def test_planted_mistakes_are_caught():
+    """Mistakes planted in the oracle's inputs on the CPU (transposed slice,
+    conjugated C, dropped imaginary term) must fail the brute-force pins."""
+    S, gSize = 5, 16
+    iu, iv, co, data, C = small_problem(6, 3, gSize, S, os_=1, nPlanes=3)
+    good = O.grid(data, iu, iv, co, C, gSize, S)
+    Ct = C.copy()
+    for c in set(co.tolist()):
+        Ct[c:c + S * S] = C[c:c + S * S].reshape(S, S).T.reshape(-1)
+    assert rel_err(O.grid(data, iu, iv, co, Ct, gSize, S), good) > 1e-3
+    assert rel_err(O.grid(data, iu, iv, co, np.conj(C), gSize, S), good) > 1e-3
+    assert rel_err(O.grid(data.real.astype(np.complex64), iu, iv, co, C, gSize, S), good) > 1e-3
+
+
+def test_cpu_baseline_runner():
+    S, gSize = 33, 256
+    iu, iv, co, data, C = small_problem(7, 400, gSize, S)
+    tg, td = O.cpu_baseline(2, 300, data, iu, iv, co, C, gSize, S)
+    assert tg > 0 and td > 0
