@@ -229,7 +229,13 @@ def test_attic_benchmark_config1_vs_reference(ctx, method):
 @pytest.mark.parametrize("rt", [1, 3, 2])
 def test_current_benchmark_vs_oracle(ctx, rt):
     """Variable per-plane widths and complex Fresnel C (runType 1), small
-    kernels with overSample 128 (runType 3), nearest neighbour (runType 2)."""
+    kernels with overSample 128 (runType 3), nearest neighbour (runType 2).
+
+    These ASKAP-layout workloads pile millions of visibilities onto the central
+    pixels, where the reference's own sequential FP32 sum is off the exact sum
+    by up to 3.5e-5 of the maximum (measured against the FP64 oracle).  So the
+    1e-5 bar is applied against the FP64 oracle, and against the FP32
+    reference the GPU must be within the reference's own error plus 1e-5."""
     meta, arrs = golden(f"current_rt{rt}")
     b = tcv.CurrentBenchmark(ctx)
     b.setRunType(rt)
@@ -237,21 +243,46 @@ def test_current_benchmark_vs_oracle(ctx, rt):
     assert b.nPixelsGridded() == meta["nPixelsGridded"]
     b.runGrid()
     g = b.grid_host()
-    gv = g.reshape(-1)[arrs["grid_pix_idx"]]
-    assert rel_err(gv, arrs["grid_pix_val"]) <= TOL
-    b.grid1.upload(np.zeros_like(g))
-    prob_iu, prob_iv = b.samples.iu.download(), b.samples.iv.download()
+    iu, iv = b.samples.iu.download(), b.samples.iv.download()
     wp, co = b.samples.wPlane.download(), b.samples.cOffset.download()
     C = b.C.download()
-    gref = O.grid(np.ones(prob_iu.size, np.complex64), prob_iu, prob_iv, co, C, b.gSize, b.sSize,
-                  wPlane=wp)
-    assert rel_err(g, gref) <= TOL
+    ones = np.ones(iu.size, np.complex64)
+    gref = O.grid(ones, iu, iv, co, C, b.gSize, b.sSize, wPlane=wp)
+    g64 = O.grid_f64(ones, iu, iv, co, C, b.gSize, b.sSize, wPlane=wp)
+    assert rel_err(g, g64) <= TOL
+    ref_err = rel_err(gref, g64)
+    assert rel_err(g, gref) <= ref_err + TOL
+    # the compiled reference's own pixels (bit-identical to gref)
+    idx = arrs["grid_pix_idx"]
+    gv = g.reshape(-1)[idx]
+    assert rel_err(gv, arrs["grid_pix_val"]) <= rel_err(arrs["grid_pix_val"], g64.reshape(-1)[idx]) + TOL
+    # degrid the reference's grid so both sides see the same input
     b.grid1.upload(gref)
     b.runDegrid()
     out = b.outdata_host()
-    oref = O.degrid(gref, prob_iu, prob_iv, co, C, b.sSize, wPlane=wp)
+    oref = O.degrid(gref, iu, iv, co, C, b.sSize, wPlane=wp)
     assert rel_err(out, oref) <= TOL
     assert rel_err(out[:256], arrs["outdata_head"]) <= TOL
+
+
+def test_cpp_dropin_driver(ctx):
+    """The tConvolveMPI-style C++ driver over the header-only drop-in
+    (include/tcv_benchmark.hpp) runs config 1 on the device; its grid and
+    degrid checksums match the compiled reference's."""
+    import os
+    import subprocess
+    meta, _ = golden("attic")
+    exe = os.path.join(os.path.dirname(tcv.LIB_PATH), "tconvolve_gpu")
+    for flag in ([], ["--plain"]):
+        r = subprocess.run([exe, "--checksum"] + flag, capture_output=True, text=True, timeout=120)
+        print(r.stdout)
+        assert r.returncode == 0, r.stderr
+        assert "Gridding rate" in r.stdout and "Degridding rate" in r.stdout
+        line = [ln for ln in r.stdout.splitlines() if ln.startswith("checksum")][0].split()
+        gr, gi = float(line[2]), float(line[3])
+        assert gi == 0.0
+        assert abs(gr - meta["grid_sum"][0]) <= 1e-5 * abs(meta["grid_sum"][0])
+        assert line[-1] == "0"
 
 
 def test_profiling_counts_launches(ctx):
