@@ -500,6 +500,8 @@ tcv_status tcv_plan_build(tcv_ctx *ctx, tcv_plan *p, const tcv_geometry *g)
     gd.cSize = g->cSize;
     gd.sSize = g->d_wPlane ? p->d_sSize : nullptr;
     p->cSize = g->cSize;
+    p->geom = gd;
+    p->hasGeom = true;
     return plan_build_kernels(ctx, p, gd);
 }
 
@@ -531,14 +533,227 @@ tcv_status tcv_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const float *d_data, const 
                           float *d_grid)
 {
     if (!ctx || !p || !d_data || !d_C || !d_grid) return tcv_fail(ctx, TCV_ERR_INVALID, "null argument");
-    return launch_grid_tiled(ctx, p, (const float2 *)d_data, (const float2 *)d_C, (float2 *)d_grid);
+    return launch_grid_tiled(ctx, p, d_data, d_C, (float2 *)d_grid, nullptr);
 }
 
 tcv_status tcv_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float *d_grid, const float *d_C,
                             float *d_data)
 {
     if (!ctx || !p || !d_data || !d_C || !d_grid) return tcv_fail(ctx, TCV_ERR_INVALID, "null argument");
-    return launch_degrid_tiled(ctx, p, (const float2 *)d_grid, (const float2 *)d_C, (float2 *)d_data);
+    return launch_degrid_tiled(ctx, p, (const float2 *)d_grid, d_C, (float2 *)d_data, nullptr);
+}
+
+// ---- FP16 storage mode ---------------------------------------------------------
+
+tcv_status tcv_ctable_to_half(tcv_ctx *ctx, const float *d_C, long long nTaps, int nPlanes,
+                              const long long *planeStart, long long planeTaps, void *d_Chalf,
+                              float *d_planeScale, float *planeScale)
+{
+    if (!ctx || nTaps < 0 || nPlanes < 1 || (nTaps > 0 && (!d_C || !d_Chalf || !d_planeScale)) ||
+        (!planeStart && planeTaps <= 0))
+        return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_ctable_to_half: bad argument");
+    if (!planeStart && planeTaps * nPlanes < nTaps)
+        return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_ctable_to_half: planes do not cover the table");
+    long long *d_ps = nullptr;
+    if (planeStart) {
+        if (planeStart[0] != 0 || planeStart[nPlanes] != nTaps)
+            return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_ctable_to_half: planeStart must span [0, nTaps]");
+        for (int k = 0; k < nPlanes; k++)
+            if (planeStart[k + 1] < planeStart[k])
+                return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_ctable_to_half: planeStart not sorted");
+        TCV_HIP(ctx, hipMalloc((void **)&d_ps, sizeof(long long) * (nPlanes + 1)));
+        hipError_t e = hipMemcpyAsync(d_ps, planeStart, sizeof(long long) * (nPlanes + 1),
+                                      hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(d_ps);
+            return tcv_fail(ctx, TCV_ERR_DEVICE, hipGetErrorString(e));
+        }
+    }
+    // the scale array doubles as the max-abs accumulator (float bit patterns)
+    tcv_status st = launch_ctable_to_half(ctx, (const float2 *)d_C, nTaps, nPlanes, d_ps,
+                                          planeStart ? 0 : planeTaps, (unsigned *)d_planeScale,
+                                          d_Chalf);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (d_ps) (void)hipFree(d_ps);
+    if (st != TCV_OK) return st;
+    if (e != hipSuccess) return tcv_fail(ctx, TCV_ERR_DEVICE, hipGetErrorString(e));
+    if (planeScale)
+        TCV_HIP(ctx, hipMemcpy(planeScale, d_planeScale, sizeof(float) * nPlanes, hipMemcpyDeviceToHost));
+    return TCV_OK;
+}
+
+tcv_status tcv_vis_to_half(tcv_ctx *ctx, const float *d_data, int nVis, void *d_dhalf, float *scale)
+{
+    if (!ctx || nVis < 0 || !scale || (nVis > 0 && (!d_data || !d_dhalf)))
+        return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_vis_to_half: bad argument");
+    unsigned *d_max = reinterpret_cast<unsigned *>(ctx->d_counters + 2);
+    tcv_status st = launch_vis_to_half(ctx, (const float2 *)d_data, nVis, d_max, d_dhalf);
+    if (st != TCV_OK) return st;
+    unsigned bits = 0;
+    TCV_HIP(ctx, hipMemcpyAsync(&bits, d_max, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
+    TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float f;
+    std::memcpy(&f, &bits, sizeof(f));
+    *scale = f > 0.f ? f : 1.f;
+    return TCV_OK;
+}
+
+tcv_status tcv_grid_tiled_half(tcv_ctx *ctx, tcv_plan *p, const void *d_dhalf, float dataScale,
+                               const void *d_Chalf, const float *d_planeScale, long long planeTaps,
+                               float *d_grid)
+{
+    if (!ctx || !p || !d_dhalf || !d_Chalf || !d_planeScale || !d_grid || planeTaps < 0 ||
+        planeTaps > 0x7fffffffLL)
+        return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_grid_tiled_half: bad argument");
+    const HalfStorage h{dataScale, d_planeScale, (int)planeTaps};
+    return launch_grid_tiled(ctx, p, d_dhalf, d_Chalf, (float2 *)d_grid, &h);
+}
+
+tcv_status tcv_degrid_tiled_half(tcv_ctx *ctx, tcv_plan *p, const float *d_grid, const void *d_Chalf,
+                                 const float *d_planeScale, long long planeTaps, float *d_data)
+{
+    if (!ctx || !p || !d_grid || !d_Chalf || !d_planeScale || !d_data || planeTaps < 0 ||
+        planeTaps > 0x7fffffffLL)
+        return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_degrid_tiled_half: bad argument");
+    const HalfStorage h{1.f, d_planeScale, (int)planeTaps};
+    return launch_degrid_tiled(ctx, p, (const float2 *)d_grid, d_Chalf, (float2 *)d_data, &h);
+}
+
+// ---- batching ------------------------------------------------------------------
+
+tcv_status tcv_batch_create(tcv_ctx *ctx, int nJobs, const tcv_job *jobs, tcv_batch **out)
+{
+    if (!ctx || !out || nJobs < 1 || !jobs) return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_batch_create: bad argument");
+    *out = nullptr;
+    tcv_batch *b = new tcv_batch();
+    b->nJobs = nJobs;
+    for (int j = 0; j < nJobs; j++) {
+        const tcv_job &J = jobs[j];
+        if (!J.plan || !J.d_C || !J.d_grid) {
+            delete b;
+            return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_batch_create: job " + std::to_string(j) + " incomplete");
+        }
+        if (j == 0) {
+            b->tw = J.plan->sh.tw;
+            b->th = J.plan->sh.th;
+        } else if (J.plan->sh.tw != b->tw || J.plan->sh.th != b->th) {
+            delete b;
+            return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_batch_create: all plans of a batch need one tile shape");
+        }
+        b->plans.push_back(J.plan);
+        b->host.push_back(make_job_desc(J.plan, J.d_data, J.d_C, (float2 *)J.d_grid, (float2 *)J.d_out, nullptr));
+    }
+    auto fail = [&](const char *m) {
+        if (b->d_jobs) (void)hipFree(b->d_jobs);
+        if (b->d_prefix) (void)hipFree(b->d_prefix);
+        if (b->d_queue) (void)hipFree(b->d_queue);
+        delete b;
+        return tcv_fail(ctx, TCV_ERR_NOMEM, m);
+    };
+    if (hipMalloc((void **)&b->d_jobs, sizeof(JobDesc) * nJobs) != hipSuccess ||
+        hipMalloc((void **)&b->d_prefix, sizeof(int) * (nJobs + 1)) != hipSuccess ||
+        hipMalloc((void **)&b->d_queue, 256) != hipSuccess)
+        return fail("tcv_batch_create: allocation failed");
+    if (hipMemcpy(b->d_jobs, b->host.data(), sizeof(JobDesc) * nJobs, hipMemcpyHostToDevice) != hipSuccess)
+        return fail("tcv_batch_create: upload failed");
+    *out = b;
+    return TCV_OK;
+}
+
+tcv_status tcv_batch_destroy(tcv_ctx *ctx, tcv_batch *b)
+{
+    if (!ctx) return TCV_ERR_INVALID;
+    if (!b) return TCV_OK;
+    TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    (void)hipFree(b->d_jobs);
+    (void)hipFree(b->d_prefix);
+    (void)hipFree(b->d_queue);
+    delete b;
+    return TCV_OK;
+}
+
+tcv_status tcv_batch_rebuild(tcv_ctx *ctx, tcv_batch *b)
+{
+    if (!ctx || !b) return TCV_ERR_INVALID;
+    for (tcv_plan *p : b->plans) {
+        if (!p->hasGeom) return tcv_fail(ctx, TCV_ERR_INVALID, "plan has no geometry");
+        tcv_status st = plan_build_kernels(ctx, p, p->geom);
+        if (st != TCV_OK) return st;
+    }
+    return TCV_OK;
+}
+
+tcv_status tcv_batch_grid(tcv_ctx *ctx, tcv_batch *b)
+{
+    if (!ctx || !b) return TCV_ERR_INVALID;
+    for (const JobDesc &d : b->host)
+        if (!d.data) return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_batch_grid: a job has no visibilities");
+    return launch_batch(ctx, b, false);
+}
+
+tcv_status tcv_batch_degrid(tcv_ctx *ctx, tcv_batch *b)
+{
+    if (!ctx || !b) return TCV_ERR_INVALID;
+    for (const JobDesc &d : b->host)
+        if (!d.out) return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_batch_degrid: a job has no output");
+    return launch_batch(ctx, b, true);
+}
+
+static tcv_status batch_enqueue(tcv_ctx *ctx, tcv_batch *b, int ops)
+{
+    tcv_status st = TCV_OK;
+    if ((ops & 1) && (st = tcv_batch_rebuild(ctx, b)) != TCV_OK) return st;
+    if ((ops & 2) && (st = tcv_batch_grid(ctx, b)) != TCV_OK) return st;
+    if ((ops & 4) && (st = tcv_batch_degrid(ctx, b)) != TCV_OK) return st;
+    return st;
+}
+
+tcv_status tcv_batch_run_graph(tcv_ctx *ctx, tcv_batch *b, int ops)
+{
+    if (!ctx || !b || ops < 1 || ops > 7) return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_batch_run_graph: bad argument");
+    if (!b->graph || b->graphOps != ops) {
+        if (b->graph) {
+            (void)hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+        }
+        const bool prof = ctx->profiling;
+        ctx->profiling = false;   // no event records inside the capture
+        TCV_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        tcv_status st = batch_enqueue(ctx, b, ops);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        ctx->profiling = prof;
+        if (st != TCV_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (e != hipSuccess) return tcv_fail(ctx, TCV_ERR_DEVICE, std::string("capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            b->graph = nullptr;
+            return tcv_fail(ctx, TCV_ERR_DEVICE, std::string("instantiate: ") + hipGetErrorString(e));
+        }
+        b->graphOps = ops;
+    }
+    ProfScope ps(ctx, "batch_graph");
+    TCV_HIP(ctx, hipGraphLaunch(b->graph, ctx->stream));
+    return TCV_OK;
+}
+
+// ---- C(w) on the device -----------------------------------------------------------
+
+tcv_status tcv_attic_initC_device(tcv_ctx *ctx, double freq0, double cellSize, int wSize, int baseline,
+                                  int overSample, int *support, double *wCellSize, float *d_C)
+{
+    if (!ctx || !support || !wCellSize || wSize < 1 || overSample < 1 || cellSize <= 0 || !d_C)
+        return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_attic_initC_device: bad argument");
+    tcv_status st = tcv_attic_initC(freq0, cellSize, wSize, baseline, overSample, support, wCellSize, nullptr);
+    if (st != TCV_OK) return st;
+    double *d_sum = reinterpret_cast<double *>(ctx->d_counters + 4);
+    return launch_attic_taps(ctx, freq0, cellSize, wSize, 2 * *support + 1, overSample, *wCellSize,
+                             (float2 *)d_C, d_sum);
 }
 
 }  // extern "C"

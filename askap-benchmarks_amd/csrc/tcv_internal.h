@@ -71,6 +71,25 @@ tcv_status launch_degrid_plain(tcv_ctx *ctx, const GeomDev &g,
 
 // ---- tiled path -------------------------------------------------------------
 
+// One job as the persistent kernels see it (passed by value for a single
+// job, or as an array in device memory for a batch).
+struct JobDesc {
+    const int4 *items;
+    const int4 *entries;
+    const int *meta;            // meta[0] = number of work items
+    const void *data;           // visibilities to grid (float2, or half2 in FP16 storage)
+    const void *C;              // taps (float2, or half2)
+    float2 *grid;
+    float2 *out;                // degridding output
+    const float *planeScale;    // FP16 storage: per-plane tap scale
+    unsigned cBytes;            // bytes of the tap table (buffer range)
+    int nTX;                    // tiles per grid row
+    int gSize;
+    int planeTaps;              // FP16: > 0 => plane = cOffset / planeTaps
+    float dataScale;            // FP16: visibility scale
+    int nVis;
+};
+
 struct PlanShape {
     int tw, th, chunk, k2, phases;
     int nTX, nTY, nTiles;
@@ -96,13 +115,48 @@ struct tcv_plan {
     int *scanTmp = nullptr;       // scratch for the multi-level scan
     long long scanTmpLen = 0;
     long long cSize = 0;
+    GeomDev geom{};               // geometry of the last build (for batch rebuilds)
+    bool hasGeom = false;
+};
+
+struct tcv_batch {
+    int nJobs = 0;
+    int tw = 0, th = 0;
+    bool half = false;
+    std::vector<JobDesc> host;
+    std::vector<tcv_plan *> plans;
+    JobDesc *d_jobs = nullptr;
+    int *d_prefix = nullptr;      // nJobs + 1
+    int *d_queue = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graphOps = 0;
+};
+
+// FP16 storage mode: taps and visibilities as half (re, im) pairs.
+struct HalfStorage {
+    float dataScale;            // visibility value = half * dataScale (gridding)
+    const float *planeScale;    // device, per w-plane: tap value = half * planeScale[plane]
+    int planeTaps;              // > 0: plane = cOffset / planeTaps; 0: plane stored in the plan
 };
 
 tcv_status plan_build_kernels(tcv_ctx *ctx, tcv_plan *p, const GeomDev &g);
-tcv_status launch_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *data,
-                             const float2 *C, float2 *grid);
-tcv_status launch_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *grid,
-                               const float2 *C, float2 *out);
+// half == nullptr: FP32 taps and data
+JobDesc make_job_desc(const tcv_plan *p, const void *data, const void *C, float2 *grid,
+                      float2 *out, const HalfStorage *half);
+tcv_status launch_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const void *data, const void *C,
+                             float2 *grid, const HalfStorage *half);
+tcv_status launch_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *grid, const void *C,
+                               float2 *out, const HalfStorage *half);
+// one persistent launch over every job of the batch (prefix sum computed on the device)
+tcv_status launch_batch(tcv_ctx *ctx, tcv_batch *b, bool degrid);
+// attic C(w) taps in FP64 on the device, normalised by the FP64 sum of |C|
+tcv_status launch_attic_taps(tcv_ctx *ctx, double freq0, double cellSize, int wSize, int sSize,
+                             int overSample, double wCellSize, float2 *C, double *d_sum);
+tcv_status launch_ctable_to_half(tcv_ctx *ctx, const float2 *C, long long nTaps, int nPlanes,
+                                 const long long *d_planeStart, long long planeTaps,
+                                 unsigned *d_maxbits, void *out);
+tcv_status launch_vis_to_half(tcv_ctx *ctx, const float2 *data, long long n, unsigned *d_maxbits,
+                              void *out);
 long long scan_tmp_len(long long n);
 // Supported tile configurations (tile_w, tile_h) of the tiled kernels.
 bool tiled_config_supported(int tw, int th);

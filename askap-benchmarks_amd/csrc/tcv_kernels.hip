@@ -338,7 +338,8 @@ __global__ __launch_bounds__(256) void k_plan_bin(int pass, GeomDev g, PlanDevAr
             const int key2 = (int)(((long long)vg.c0 * a.k2) / a.cSize);
             const int tx0 = vg.u0 / a.tw, tx1 = (vg.u0 + vg.s - 1) / a.tw;
             const int ty0 = vg.v0 / a.th, ty1 = (vg.v0 + vg.s - 1) / a.th;
-            const int4 ent = make_int4(vg.u0 | (vg.v0 << 16), vg.c0, d, vg.s);
+            const int plane = g.wPlane ? g.wPlane[d] : 0;
+            const int4 ent = make_int4(vg.u0 | (vg.v0 << 16), vg.c0, d, vg.s | (plane << 16));
             for (int ty = ty0; ty <= ty1; ++ty)
                 for (int tx = tx0; tx <= tx1; ++tx) {
                     const long long key = (long long)(ty * a.nTX + tx) * a.k2 + key2;
@@ -400,6 +401,85 @@ __global__ void k_plan_meta(const int *__restrict__ itemStart, int nItemSlots,
 
 __global__ void k_reset_queue(int *meta) { if (threadIdx.x == 0) meta[1] = 0; }
 
+// ---- tap storage ---------------------------------------------------------------
+// FP32: complex64 taps (8 B).  FP16: (re, im) half pairs (4 B) scaled by the
+// max-abs of their w-plane; accumulation stays FP32.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+template <bool HALF>
+struct Taps;
+
+template <>
+struct Taps<false> {
+    static constexpr unsigned kBytes = 8;
+    __device__ static __forceinline__ float2 load(__amdgpu_buffer_rsrc_t r, unsigned voff)
+    {
+        return load_c(r, voff);
+    }
+};
+
+template <>
+struct Taps<true> {
+    static constexpr unsigned kBytes = 4;
+    __device__ static __forceinline__ float2 load(__amdgpu_buffer_rsrc_t r, unsigned voff)
+    {
+        const half2_t h = __builtin_bit_cast(half2_t, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
+        return make_float2((float)h.x, (float)h.y);
+    }
+};
+
+__device__ __forceinline__ int entry_width(int w) { return w & 0xFFFF; }
+__device__ __forceinline__ int entry_plane(const int4 &en, const JobDesc &d)
+{
+    return d.planeTaps > 0 ? en.y / d.planeTaps : (int)((unsigned)en.w >> 16);
+}
+
+template <bool HALF>
+__device__ __forceinline__ float2 vis_value(const JobDesc &d, const int4 &en)
+{
+    if constexpr (!HALF) {
+        return static_cast<const float2 *>(d.data)[en.z];
+    } else {
+        const half2_t v = static_cast<const half2_t *>(d.data)[en.z];
+        const float sc = d.dataScale * d.planeScale[entry_plane(en, d)];
+        return make_float2((float)v.x * sc, (float)v.y * sc);
+    }
+}
+
+// Work-item source of the persistent kernels: one job (descriptor passed by
+// value) or a batch (descriptors in device memory, located through the
+// exclusive prefix sum of the jobs' item counts).
+template <bool BATCH>
+struct ItemSource {
+    JobDesc one;
+    const JobDesc *jobs;
+    const int *prefix;
+    int nJobs;
+    int *queue;
+
+    __device__ __forceinline__ int total() const
+    {
+        if constexpr (BATCH) return __builtin_amdgcn_readfirstlane(prefix[nJobs]);
+        else return __builtin_amdgcn_readfirstlane(one.meta[0]);
+    }
+    // item -> (job descriptor, local item index); wave-uniform
+    __device__ __forceinline__ void locate(int item, JobDesc &d, int &local) const
+    {
+        if constexpr (BATCH) {
+            int lo = 0, hi = nJobs;   // prefix[lo] <= item < prefix[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (__builtin_amdgcn_readfirstlane(prefix[mid]) <= item) lo = mid; else hi = mid;
+            }
+            d = jobs[lo];
+            local = item - __builtin_amdgcn_readfirstlane(prefix[lo]);
+        } else {
+            d = one;
+            local = item;
+        }
+    }
+};
+
 // ---- tiled gridder ------------------------------------------------------------
 // attic/tConvolveMPI/Benchmark.cc:131-154 reorganised as owner-computes: a
 // workgroup owns a TW x TH grid tile held in VGPRs.  Lane l covers column
@@ -407,31 +487,34 @@ __global__ void k_reset_queue(int *meta) { if (threadIdx.x == 0) meta[1] = 0; }
 // consecutive row groups.  For every binned visibility the workgroup streams
 // the part of its C slice that falls on the tile, accumulates, and writes the
 // tile back once.  Persistent: workgroups pull items from a device queue.
-template <int TW, int R, int NW, int RB>
-__global__ __launch_bounds__(NW * 64) void k_grid_tiled(
-    const int4 *__restrict__ items, const int4 *__restrict__ entries,
-    int *__restrict__ meta, int nTX, const float2 *__restrict__ data,
-    const float2 *__restrict__ C, unsigned cBytes, float2 *__restrict__ grid, int gSize)
+template <int TW, int R, int NW, int RB, bool HALF, bool BATCH>
+__global__ __launch_bounds__(NW * 64) void k_grid_tiled(ItemSource<BATCH> src)
 {
     constexpr int RPI = 64 / TW;
     constexpr int TH = NW * R * RPI;
+    constexpr unsigned TB = Taps<HALF>::kBytes;
     static_assert(R % RB == 0, "row block must divide the register rows");
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lx = lane % TW, ly = lane / TW;
-    const __amdgpu_buffer_rsrc_t rc = make_rsrc(C, cBytes);
     __shared__ int s_item;
-    const int nItems = __builtin_amdgcn_readfirstlane(meta[0]);
+    const int nItems = src.total();
     const int q0 = wave * R;   // first row group of this wave
 
     for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(&meta[1], 1);
+        if (threadIdx.x == 0) s_item = atomicAdd(src.queue, 1);
         __syncthreads();
         const int item = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
         if (item >= nItems) break;
-        const int4 it = items[item];
-        const int tyi = it.x / nTX, txi = it.x - tyi * nTX;
+        JobDesc jd;
+        int local;
+        src.locate(item, jd, local);
+        const __amdgpu_buffer_rsrc_t rc = make_rsrc(jd.C, jd.cBytes);
+        const int4 *__restrict__ entries = jd.entries;
+        const int gSize = jd.gSize;
+        const int4 it = jd.items[local];
+        const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
         const int Bx = txi * TW, By = tyi * TH;
         const int e0 = it.y, e1 = it.y + it.z;
 
@@ -441,17 +524,17 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled(
 
         int4 enA = entries[e0];
         int4 enB = entries[min(e0 + 1, e1 - 1)];
-        float2 dA = data[enA.z];
+        float2 dA = vis_value<HALF>(jd, enA);
         for (int e = e0; e < e1; ++e) {
-            const float2 dB = data[enB.z];
+            const float2 dB = vis_value<HALF>(jd, enB);
             const int4 enC = entries[min(e + 2, e1 - 1)];
 
             const int u0 = enA.x & 0xFFFF, v0 = (int)((unsigned)enA.x >> 16);
-            const int c0 = enA.y, s = enA.w;
+            const int c0 = enA.y, s = entry_width(enA.w);
             const int cx = Bx + lx - u0;
             const int dy = By - v0;   // C row of tile row r is r + dy
             const unsigned lanev = ((unsigned)cx < (unsigned)s)
-                                       ? (unsigned)(c0 + cx) * 8u : kOOB;
+                                       ? (unsigned)(c0 + cx) * TB : kOOB;
 #pragma unroll
             for (int jb = 0; jb < R; jb += RB) {
                 const int r0 = (q0 + jb) * RPI;
@@ -461,8 +544,8 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled(
                     for (int k = 0; k < RB; ++k) {
                         const int cy = r0 + k * RPI + ly + dy;
                         const unsigned vo = ((unsigned)cy < (unsigned)s)
-                                                ? lanev + (unsigned)(cy * s) * 8u : kOOB;
-                        cv[k] = load_c(rc, vo);
+                                                ? lanev + (unsigned)(cy * s) * TB : kOOB;
+                        cv[k] = Taps<HALF>::load(rc, vo);
                     }
 #pragma unroll
                     for (int k = 0; k < RB; ++k) {
@@ -481,7 +564,7 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled(
         for (int j = 0; j < R; ++j) {
             const int row = By + (q0 + j) * RPI + ly;
             if (row < gSize && col < gSize && (ar[j] != 0.f || ai[j] != 0.f)) {
-                float2 *gp = grid + (size_t)row * gSize + col;
+                float2 *gp = jd.grid + (size_t)row * gSize + col;
                 if (it.w) {
                     float2 o = *gp;
                     o.x += ar[j];
@@ -502,30 +585,34 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled(
 // workgroup split the tile's visibilities.  Each visibility's partial sum over
 // the tile is reduced across the wave with DPP and added to the output with
 // two FP32 atomics (the output is zeroed first: degridding overwrites).
-template <int TW, int RD, int NW, int RB>
-__global__ __launch_bounds__(NW * 64) void k_degrid_tiled(
-    const int4 *__restrict__ items, const int4 *__restrict__ entries,
-    int *__restrict__ meta, int nTX, const float2 *__restrict__ grid, int gSize,
-    const float2 *__restrict__ C, unsigned cBytes, float2 *__restrict__ out)
+template <int TW, int RD, int NW, int RB, bool HALF, bool BATCH>
+__global__ __launch_bounds__(NW * 64) void k_degrid_tiled(ItemSource<BATCH> src)
 {
     constexpr int RPI = 64 / TW;
     constexpr int TH = RD * RPI;
+    constexpr unsigned TB = Taps<HALF>::kBytes;
     static_assert(RD % RB == 0, "row block must divide the register rows");
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lx = lane % TW, ly = lane / TW;
-    const __amdgpu_buffer_rsrc_t rc = make_rsrc(C, cBytes);
     __shared__ int s_item;
-    const int nItems = __builtin_amdgcn_readfirstlane(meta[0]);
+    const int nItems = src.total();
 
     for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(&meta[1], 1);
+        if (threadIdx.x == 0) s_item = atomicAdd(src.queue, 1);
         __syncthreads();
         const int item = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
         if (item >= nItems) break;
-        const int4 it = items[item];
-        const int tyi = it.x / nTX, txi = it.x - tyi * nTX;
+        JobDesc jd;
+        int local;
+        src.locate(item, jd, local);
+        const __amdgpu_buffer_rsrc_t rc = make_rsrc(jd.C, jd.cBytes);
+        const int4 *__restrict__ entries = jd.entries;
+        const float2 *__restrict__ grid = jd.grid;
+        const int gSize = jd.gSize;
+        const int4 it = jd.items[local];
+        const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
         const int Bx = txi * TW, By = tyi * TH;
         const int e0 = it.y, e1 = it.y + it.z;
         const int col = Bx + lx;
@@ -544,11 +631,11 @@ __global__ __launch_bounds__(NW * 64) void k_degrid_tiled(
         for (int e = first; e < e1; e += NW) {
             const int4 enB = entries[min(e + NW, e1 - 1)];
             const int u0 = enA.x & 0xFFFF, v0 = (int)((unsigned)enA.x >> 16);
-            const int c0 = enA.y, s = enA.w;
+            const int c0 = enA.y, s = entry_width(enA.w);
             const int cx = Bx + lx - u0;
             const int dy = By - v0;
             const unsigned lanev = ((unsigned)cx < (unsigned)s)
-                                       ? (unsigned)(c0 + cx) * 8u : kOOB;
+                                       ? (unsigned)(c0 + cx) * TB : kOOB;
             float sr = 0.f, si = 0.f;
 #pragma unroll
             for (int jb = 0; jb < RD; jb += RB) {
@@ -559,8 +646,8 @@ __global__ __launch_bounds__(NW * 64) void k_degrid_tiled(
                     for (int k = 0; k < RB; ++k) {
                         const int cy = r0 + k * RPI + ly + dy;
                         const unsigned vo = ((unsigned)cy < (unsigned)s)
-                                                ? lanev + (unsigned)(cy * s) * 8u : kOOB;
-                        cv[k] = load_c(rc, vo);
+                                                ? lanev + (unsigned)(cy * s) * TB : kOOB;
+                        cv[k] = Taps<HALF>::load(rc, vo);
                     }
 #pragma unroll
                     for (int k = 0; k < RB; ++k) {
@@ -571,13 +658,112 @@ __global__ __launch_bounds__(NW * 64) void k_degrid_tiled(
             }
             sr = wave_sum(sr);
             si = wave_sum(si);
+            if constexpr (HALF) {
+                const float ps = jd.planeScale[entry_plane(enA, jd)];
+                sr *= ps;
+                si *= ps;
+            }
             if (lane == 0) {
-                float *op = reinterpret_cast<float *>(out + enA.z);
+                float *op = reinterpret_cast<float *>(jd.out + enA.z);
                 unsafeAtomicAdd(op, sr);
                 unsafeAtomicAdd(op + 1, si);
             }
             enA = enB;
         }
+    }
+}
+
+// ---- FP16 storage conversion ----------------------------------------------------
+// Per-plane max |component| of C, then taps / scale rounded to half.  Each thread
+// walks a contiguous run of taps and flushes its running max with one atomicMax
+// (positive floats order like their bit patterns) whenever the plane changes.
+__device__ __forceinline__ int plane_of(long long t, const long long *planeStart, int nPlanes,
+                                        long long planeTaps)
+{
+    if (planeTaps > 0) return (int)(t / planeTaps);
+    int lo = 0, hi = nPlanes;   // planeStart[lo] <= t < planeStart[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (planeStart[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+constexpr int kRun = 1024;
+
+__global__ __launch_bounds__(256) void k_plane_maxabs(const float2 *__restrict__ C, long long nTaps,
+                                                      const long long *__restrict__ planeStart,
+                                                      int nPlanes, long long planeTaps,
+                                                      unsigned *__restrict__ maxbits)
+{
+    const long long t0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * kRun;
+    if (t0 >= nTaps) return;
+    const long long t1 = min(nTaps, t0 + kRun);
+    int plane = plane_of(t0, planeStart, nPlanes, planeTaps);
+    long long next = planeTaps > 0 ? (long long)(plane + 1) * planeTaps : planeStart[plane + 1];
+    float m = 0.f;
+    for (long long t = t0; t < t1; ++t) {
+        if (t >= next) {
+            atomicMax(&maxbits[plane], __float_as_uint(m));
+            m = 0.f;
+            plane = plane_of(t, planeStart, nPlanes, planeTaps);
+            next = planeTaps > 0 ? (long long)(plane + 1) * planeTaps : planeStart[plane + 1];
+        }
+        const float2 c = C[t];
+        m = fmaxf(m, fmaxf(fabsf(c.x), fabsf(c.y)));
+    }
+    atomicMax(&maxbits[plane], __float_as_uint(m));
+}
+
+__global__ __launch_bounds__(256) void k_taps_to_half(const float2 *__restrict__ C, long long nTaps,
+                                                      const long long *__restrict__ planeStart,
+                                                      int nPlanes, long long planeTaps,
+                                                      unsigned *__restrict__ maxbits,
+                                                      half2_t *__restrict__ out)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; t < nTaps; t += stride) {
+        const int p = plane_of(t, planeStart, nPlanes, planeTaps);
+        float sc = __uint_as_float(maxbits[p]);
+        if (!(sc > 0.f)) sc = 1.f;
+        const float2 c = C[t];
+        half2_t hv;
+        hv.x = (_Float16)(c.x / sc);
+        hv.y = (_Float16)(c.y / sc);
+        out[t] = hv;
+    }
+}
+
+// max |component| over n complex values (one atomicMax per wave)
+__global__ __launch_bounds__(256) void k_maxabs(const float2 *__restrict__ v, long long n,
+                                                unsigned *__restrict__ maxbits)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    float m = 0.f;
+    for (; i < n; i += stride) {
+        const float2 c = v[i];
+        m = fmaxf(m, fmaxf(fabsf(c.x), fabsf(c.y)));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(maxbits, __float_as_uint(m));
+}
+
+__global__ __launch_bounds__(256) void k_vis_to_half(const float2 *__restrict__ v, long long n,
+                                                     const unsigned *__restrict__ maxbits,
+                                                     half2_t *__restrict__ out)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    float sc = __uint_as_float(*maxbits);
+    if (!(sc > 0.f)) sc = 1.f;
+    for (; i < n; i += stride) {
+        const float2 c = v[i];
+        half2_t hv;
+        hv.x = (_Float16)(c.x / sc);
+        hv.y = (_Float16)(c.y / sc);
+        out[i] = hv;
     }
 }
 
@@ -742,36 +928,40 @@ int persistent_blocks(tcv_ctx *ctx, const void *kernel, int bs)
     return std::max(1, ctx->cus) * per;
 }
 
-template <int TW, int TH>
-tcv_status grid_tiled_t(tcv_ctx *ctx, tcv_plan *p, const float2 *data, const float2 *C,
-                        float2 *grid)
+__global__ void k_batch_prefix(const JobDesc *__restrict__ jobs, int nJobs, int *__restrict__ prefix,
+                               int *__restrict__ queue)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        int run = 0;
+        for (int j = 0; j < nJobs; ++j) {
+            prefix[j] = run;
+            run += jobs[j].meta[0];
+        }
+        prefix[nJobs] = run;
+        *queue = 0;
+    }
+}
+
+template <int TW, int TH, bool HALF, bool BATCH>
+tcv_status grid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src)
 {
     using T = TileCfg<TW, TH>;
-    auto kern = k_grid_tiled<TW, T::RG, kNW, (T::RG % kRB == 0 ? kRB : T::RG)>;
+    auto kern = k_grid_tiled<TW, T::RG, kNW, (T::RG % kRB == 0 ? kRB : T::RG), HALF, BATCH>;
     const int blocks = persistent_blocks(ctx, (const void *)kern, kNW * 64);
-    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, p->meta);
-    ProfScope ps(ctx, "grid_tiled");
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kNW * 64), 0, ctx->stream,
-                       (const int4 *)p->items, (const int4 *)p->entries, p->meta, p->sh.nTX,
-                       data, C, (unsigned)(p->cSize * 8), grid, p->gSize);
+    ProfScope ps(ctx, BATCH ? "grid_batch" : (HALF ? "grid_tiled_half" : "grid_tiled"));
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kNW * 64), 0, ctx->stream, src);
     TCV_HIP(ctx, hipGetLastError());
     return TCV_OK;
 }
 
-template <int TW, int TH>
-tcv_status degrid_tiled_t(tcv_ctx *ctx, tcv_plan *p, const float2 *grid, const float2 *C,
-                          float2 *out)
+template <int TW, int TH, bool HALF, bool BATCH>
+tcv_status degrid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src)
 {
     using T = TileCfg<TW, TH>;
-    auto kern = k_degrid_tiled<TW, T::RD, kNW, (T::RD % kRB == 0 ? kRB : T::RD)>;
+    auto kern = k_degrid_tiled<TW, T::RD, kNW, (T::RD % kRB == 0 ? kRB : T::RD), HALF, BATCH>;
     const int blocks = persistent_blocks(ctx, (const void *)kern, kNW * 64);
-    tcv_status st = launch_fill_u32(ctx, out, 0, (size_t)p->nVis * 2);
-    if (st != TCV_OK) return st;
-    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, p->meta);
-    ProfScope ps(ctx, "degrid_tiled");
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kNW * 64), 0, ctx->stream,
-                       (const int4 *)p->items, (const int4 *)p->entries, p->meta, p->sh.nTX,
-                       grid, p->gSize, C, (unsigned)(p->cSize * 8), out);
+    ProfScope ps(ctx, BATCH ? "degrid_batch" : (HALF ? "degrid_tiled_half" : "degrid_tiled"));
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kNW * 64), 0, ctx->stream, src);
     TCV_HIP(ctx, hipGetLastError());
     return TCV_OK;
 }
@@ -789,20 +979,177 @@ bool tiled_config_supported(int tw, int th)
     return false;
 }
 
-tcv_status launch_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *data, const float2 *C,
-                             float2 *grid)
+JobDesc make_job_desc(const tcv_plan *p, const void *data, const void *C, float2 *grid,
+                      float2 *out, const HalfStorage *half)
 {
-#define X(a, b) if (p->sh.tw == a && p->sh.th == b) return grid_tiled_t<a, b>(ctx, p, data, C, grid);
+    JobDesc d{};
+    d.items = p->items;
+    d.entries = p->entries;
+    d.meta = p->meta;
+    d.data = data;
+    d.C = C;
+    d.grid = grid;
+    d.out = out;
+    d.planeScale = half ? half->planeScale : nullptr;
+    d.cBytes = (unsigned)(p->cSize * (half ? 4 : 8));
+    d.nTX = p->sh.nTX;
+    d.gSize = p->gSize;
+    d.planeTaps = half ? half->planeTaps : 0;
+    d.dataScale = half ? half->dataScale : 1.f;
+    d.nVis = p->nVis;
+    return d;
+}
+
+tcv_status launch_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const void *data, const void *C,
+                             float2 *grid, const HalfStorage *half)
+{
+    ItemSource<false> src{make_job_desc(p, data, C, grid, nullptr, half), nullptr, nullptr, 0,
+                          p->meta + 1};
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, p->meta);
+    TCV_HIP(ctx, hipGetLastError());
+#define X(a, b)                                                                        \
+    if (p->sh.tw == a && p->sh.th == b)                                                \
+        return half ? grid_tiled_t<a, b, true, false>(ctx, src)                        \
+                    : grid_tiled_t<a, b, false, false>(ctx, src);
     TCV_TILE_CONFIGS(X)
 #undef X
     return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");
 }
 
-tcv_status launch_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *grid, const float2 *C,
-                               float2 *out)
+tcv_status launch_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *grid, const void *C,
+                               float2 *out, const HalfStorage *half)
 {
-#define X(a, b) if (p->sh.tw == a && p->sh.th == b) return degrid_tiled_t<a, b>(ctx, p, grid, C, out);
+    ItemSource<false> src{make_job_desc(p, nullptr, C, const_cast<float2 *>(grid), out, half),
+                          nullptr, nullptr, 0, p->meta + 1};
+    tcv_status st = launch_fill_u32(ctx, out, 0, (size_t)p->nVis * 2);
+    if (st != TCV_OK) return st;
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, p->meta);
+    TCV_HIP(ctx, hipGetLastError());
+#define X(a, b)                                                                        \
+    if (p->sh.tw == a && p->sh.th == b)                                                \
+        return half ? degrid_tiled_t<a, b, true, false>(ctx, src)                      \
+                    : degrid_tiled_t<a, b, false, false>(ctx, src);
     TCV_TILE_CONFIGS(X)
 #undef X
     return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");
+}
+
+tcv_status launch_batch(tcv_ctx *ctx, tcv_batch *b, bool degrid)
+{
+    if (degrid)
+        for (const JobDesc &d : b->host) {
+            tcv_status st = launch_fill_u32(ctx, d.out, 0, (size_t)d.nVis * 2);
+            if (st != TCV_OK) return st;
+        }
+    hipLaunchKernelGGL(k_batch_prefix, dim3(1), dim3(64), 0, ctx->stream,
+                       (const JobDesc *)b->d_jobs, b->nJobs, b->d_prefix, b->d_queue);
+    TCV_HIP(ctx, hipGetLastError());
+    ItemSource<true> src{JobDesc{}, b->d_jobs, b->d_prefix, b->nJobs, b->d_queue};
+    // batches run FP32 storage
+#define X(a, c)                                                                        \
+    if (b->tw == a && b->th == c)                                                      \
+        return degrid ? degrid_tiled_t<a, c, false, true>(ctx, src)                    \
+                      : grid_tiled_t<a, c, false, true>(ctx, src);
+    TCV_TILE_CONFIGS(X)
+#undef X
+    return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");
+}
+
+tcv_status launch_ctable_to_half(tcv_ctx *ctx, const float2 *C, long long nTaps, int nPlanes,
+                                 const long long *d_planeStart, long long planeTaps,
+                                 unsigned *d_maxbits, void *out)
+{
+    if (nTaps <= 0) return TCV_OK;
+    tcv_status st = launch_fill_u32(ctx, d_maxbits, 0, (size_t)nPlanes);
+    if (st != TCV_OK) return st;
+    const long long runs = (nTaps + kRun - 1) / kRun;
+    hipLaunchKernelGGL(k_plane_maxabs, dim3((unsigned)((runs + 255) / 256)), dim3(256), 0,
+                       ctx->stream, C, nTaps, d_planeStart, nPlanes, planeTaps, d_maxbits);
+    TCV_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_taps_to_half, dim3(grid_blocks((size_t)nTaps, 256)), dim3(256), 0,
+                       ctx->stream, C, nTaps, d_planeStart, nPlanes, planeTaps, d_maxbits,
+                       (half2_t *)out);
+    TCV_HIP(ctx, hipGetLastError());
+    return TCV_OK;
+}
+
+tcv_status launch_vis_to_half(tcv_ctx *ctx, const float2 *data, long long n, unsigned *d_maxbits,
+                              void *out)
+{
+    tcv_status st = launch_fill_u32(ctx, d_maxbits, 0, 1);
+    if (st != TCV_OK || n <= 0) return st;
+    hipLaunchKernelGGL(k_maxabs, dim3(std::min(grid_blocks((size_t)n, 256), 1024)), dim3(256), 0,
+                       ctx->stream, data, n, d_maxbits);
+    TCV_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_vis_to_half, dim3(grid_blocks((size_t)n, 256)), dim3(256), 0, ctx->stream,
+                       data, n, (const unsigned *)d_maxbits, (half2_t *)out);
+    TCV_HIP(ctx, hipGetLastError());
+    return TCV_OK;
+}
+
+// ---- attic C(w) taps on the device ----------------------------------------------
+// attic/tConvolveMPI/Benchmark.cc:236-269 evaluated per tap in FP64 on the GPU.
+// The normalisation divides by the FP64 sum of |C| (the reference accumulates
+// that sum in float, which stops growing near 2^24 on large tables); use the
+// host tcv_attic_initC for tables bit-identical to the reference.
+namespace {
+
+__global__ __launch_bounds__(256) void k_attic_taps(double freq0, double cellSize, int wSize, int s,
+                                                    int os, double wCellSize, float2 *__restrict__ C,
+                                                    double *__restrict__ sum)
+{
+    const long long n = (long long)s * s * os * os * wSize;
+    const int cc = (s - 1) / 2;
+    double acc = 0.0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x) {
+        long long q = t;
+        const int i = (int)(q % s); q /= s;
+        const int j = (int)(q % s); q /= s;
+        const int osi = (int)(q % os); q /= os;
+        const int osj = (int)(q % os); q /= os;
+        const int k = (int)q;
+        const double w = (double)(k - wSize / 2);
+        const double tj = (double)(j - cc) + (double)osj / (double)os;
+        const double ti = (double)(i - cc) + (double)osi / (double)os;
+        const double r2 = tj * tj + ti * ti;
+        float v;
+        if (w != 0.0) {
+            const double fScale = sqrt(fabs(w) * wCellSize * freq0) / cellSize;
+            v = (float)cos(r2 / (w * fScale));
+        } else {
+            v = (float)exp(-r2);
+        }
+        C[t] = make_float2(v, 0.f);
+        acc += fabs((double)v);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(sum, acc);
+}
+
+__global__ __launch_bounds__(256) void k_scale_taps(float2 *__restrict__ C, long long n, double target,
+                                                    const double *__restrict__ sum)
+{
+    const float sc = (float)(target / *sum);
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x) {
+        C[t].x *= sc;
+    }
+}
+
+}  // namespace
+
+tcv_status launch_attic_taps(tcv_ctx *ctx, double freq0, double cellSize, int wSize, int sSize,
+                             int overSample, double wCellSize, float2 *C, double *d_sum)
+{
+    const long long n = (long long)sSize * sSize * overSample * overSample * wSize;
+    tcv_status st = launch_fill_u32(ctx, d_sum, 0, 2);
+    if (st != TCV_OK) return st;
+    hipLaunchKernelGGL(k_attic_taps, dim3(2048), dim3(256), 0, ctx->stream, freq0, cellSize, wSize,
+                       sSize, overSample, wCellSize, C, d_sum);
+    TCV_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_scale_taps, dim3(2048), dim3(256), 0, ctx->stream, C, n,
+                       (double)wSize * overSample * overSample, (const double *)d_sum);
+    TCV_HIP(ctx, hipGetLastError());
+    return TCV_OK;
 }

@@ -33,6 +33,10 @@ class Geometry(ct.Structure):
     ]
 
 
+class Job(ct.Structure):
+    _fields_ = [("plan", P), ("d_data", P), ("d_C", P), ("d_grid", P), ("d_out", P)]
+
+
 class Tuning(ct.Structure):
     _fields_ = [("tile_w", I), ("tile_h", I), ("chunk", I), ("slice_keys", I),
                 ("phases", I)]
@@ -72,6 +76,17 @@ SIGS = {
     "tcv_plan_info": (I, [P, P, ct.POINTER(Tuning), PLL, PLL, PLL]),
     "tcv_grid_tiled": (I, [P, P, P, P, P]),
     "tcv_degrid_tiled": (I, [P, P, P, P, P]),
+    "tcv_ctable_to_half": (I, [P, P, LL, I, P, LL, P, P, P]),
+    "tcv_vis_to_half": (I, [P, P, I, P, ct.POINTER(ct.c_float)]),
+    "tcv_grid_tiled_half": (I, [P, P, P, ct.c_float, P, P, LL, P]),
+    "tcv_degrid_tiled_half": (I, [P, P, P, P, P, LL, P]),
+    "tcv_batch_create": (I, [P, I, ct.POINTER(Job), ct.POINTER(P)]),
+    "tcv_batch_destroy": (I, [P, P]),
+    "tcv_batch_rebuild": (I, [P, P]),
+    "tcv_batch_grid": (I, [P, P]),
+    "tcv_batch_degrid": (I, [P, P]),
+    "tcv_batch_run_graph": (I, [P, P, I]),
+    "tcv_attic_initC_device": (I, [P, D, D, I, I, I, PI, PD, P]),
 }
 
 _lib = None

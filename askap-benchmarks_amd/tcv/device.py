@@ -5,7 +5,7 @@ import ctypes as ct
 
 import numpy as np
 
-from ._lib import STATUS, Geometry, Tuning, lib
+from ._lib import STATUS, Geometry, Job, Tuning, lib
 
 P = ct.c_void_p
 
@@ -231,6 +231,16 @@ class Plan:
         self.ctx.check(lib().tcv_degrid_tiled(self.ctx.h, self.h, grid.ptr, C.ptr, out.ptr),
                        "degrid_tiled")
 
+    def grid_half(self, data: "HalfVis", C: "HalfTable", grid: DeviceArray):
+        self.ctx.check(lib().tcv_grid_tiled_half(self.ctx.h, self.h, data.buf.ptr, data.scale,
+                                                 C.buf.ptr, C.scales.ptr, C.planeTaps, grid.ptr),
+                       "grid_tiled_half")
+
+    def degrid_half(self, grid: DeviceArray, C: "HalfTable", out: DeviceArray):
+        self.ctx.check(lib().tcv_degrid_tiled_half(self.ctx.h, self.h, grid.ptr, C.buf.ptr,
+                                                   C.scales.ptr, C.planeTaps, out.ptr),
+                       "degrid_tiled_half")
+
     def close(self):
         if getattr(self, "h", None) and self.ctx.h:
             lib().tcv_plan_destroy(self.ctx.h, self.h)
@@ -241,3 +251,91 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class HalfTable:
+    """C(w) in FP16 storage: half (re, im) pairs with one max-abs scale per
+    w-plane.  planeTaps > 0: uniform planes (attic layout); otherwise the plane
+    boundaries come from planeStart (nPlanes + 1 tap offsets)."""
+
+    def __init__(self, ctx: Context, C: DeviceArray, nPlanes: int, planeTaps: int = 0,
+                 planeStart=None):
+        self.ctx = ctx
+        n = C.size
+        self.buf = DeviceArray(ctx, n * 2, np.float16)
+        self.scales = DeviceArray(ctx, nPlanes, np.float32)
+        self.planeTaps = int(planeTaps) if planeStart is None else 0
+        ps = None if planeStart is None else np.ascontiguousarray(planeStart, np.int64)
+        host = np.empty(nPlanes, np.float32)
+        ctx.check(lib().tcv_ctable_to_half(ctx.h, C.ptr, n, nPlanes, _p(ps), int(planeTaps),
+                                           self.buf.ptr, self.scales.ptr, _p(host)),
+                  "ctable_to_half")
+        self.host_scales = host
+
+
+class HalfVis:
+    """Visibilities in FP16 storage (half pairs and one max-abs scale)."""
+
+    def __init__(self, ctx: Context, data: DeviceArray):
+        self.buf = DeviceArray(ctx, data.size * 2, np.float16)
+        sc = ct.c_float()
+        ctx.check(lib().tcv_vis_to_half(ctx.h, data.ptr, data.size, self.buf.ptr, ct.byref(sc)),
+                  "vis_to_half")
+        self.scale = sc.value
+
+
+class Batch:
+    """Many jobs in one persistent launch over a device work queue (tcv_batch).
+
+    jobs: sequence of (plan, data, C, grid, out) with DeviceArrays (data or
+    out may be None).  All plans must share one tile shape."""
+
+    def __init__(self, ctx: Context, jobs):
+        self.ctx = ctx
+        self._keep = list(jobs)
+        arr = (Job * len(jobs))()
+        for k, (plan, data, C, grid, out) in enumerate(jobs):
+            arr[k] = Job(plan.h, data.ptr if data is not None else None, C.ptr, grid.ptr,
+                         out.ptr if out is not None else None)
+        h = P()
+        ctx.check(lib().tcv_batch_create(ctx.h, len(jobs), arr, ct.byref(h)), "batch_create")
+        self.h = h
+
+    def rebuild(self):
+        self.ctx.check(lib().tcv_batch_rebuild(self.ctx.h, self.h), "batch_rebuild")
+
+    def grid(self):
+        self.ctx.check(lib().tcv_batch_grid(self.ctx.h, self.h), "batch_grid")
+
+    def degrid(self):
+        self.ctx.check(lib().tcv_batch_degrid(self.ctx.h, self.h), "batch_degrid")
+
+    def run_graph(self, ops: int):
+        """ops: 1 rebuild + 2 grid + 4 degrid; captured once, replayed after."""
+        self.ctx.check(lib().tcv_batch_run_graph(self.ctx.h, self.h, ops), "batch_run_graph")
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib().tcv_batch_destroy(self.ctx.h, self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def attic_initC_device(ctx: Context, freq0: float, cellSize: float, wSize: int, baseline: int,
+                       overSample: int = 8):
+    """attic C(w) generated on the device (FP64 taps, FP64-sum normalisation).
+    Returns (support, wCellSize, DeviceArray of complex64 taps)."""
+    sup, wcs = ct.c_int(), ct.c_double()
+    from .setup import attic_initC
+    s0, _, _ = attic_initC(freq0, cellSize, wSize, baseline, overSample, table=False)
+    S = 2 * s0 + 1
+    C = DeviceArray(ctx, S * S * overSample * overSample * wSize, np.complex64)
+    ctx.check(lib().tcv_attic_initC_device(ctx.h, freq0, cellSize, wSize, int(baseline), overSample,
+                                           ct.byref(sup), ct.byref(wcs), C.ptr),
+              "attic_initC_device")
+    return sup.value, wcs.value, C

@@ -101,3 +101,71 @@ class GridJob:
 
 
 __all__ = ["init_offsets", "GridJob", "TcvError"]
+
+
+class BatchJobs:
+    """A set of independent jobs resident on the device (the uneven batch).
+
+    Each job gets its own grid, C(w) table (generated on the device), data,
+    offsets and plan; run_* execute the whole set one way or another so the
+    three execution modes can be compared on identical inputs:
+      plain    one launch per job of the one-workgroup-per-visibility kernels
+      tiled    one tiled (binned) launch per job
+      batched  one persistent launch over all jobs' work items, captured in a
+               hipGraph together with the re-binning
+    """
+
+    def __init__(self, ctx: Context, specs, tuning: dict | None = None):
+        from .device import Batch, attic_initC_device
+        self.ctx = ctx
+        self.specs = specs
+        self.jobs = []
+        tuning = dict(tuning or {})
+        tuning.setdefault("tile_w", 64)
+        for sp in specs:
+            sup, wcs, C = attic_initC_device(ctx, float(sp["freq"][0]), sp["cellSize"], sp["wSize"],
+                                             int(sp["baseline"]))
+            assert 2 * sup + 1 == sp["sSize"], (sup, sp["sSize"])
+            du, dv, dw = (ctx.to_device(sp[k]) for k in ("u", "v", "w"))
+            iu, iv, _, co, npp, bad = init_offsets(ctx, du, dv, dw, sp["freq"], sp["cellSize"], wcs,
+                                                   sp["wSize"], sp["gSize"], 8, [sp["sSize"]], None,
+                                                   want_wplane=False)
+            if bad:
+                raise ValueError(f"job {sp['job']}: {bad} visibilities out of range")
+            vis = Visibilities(ctx, iu, iv, co, [sp["sSize"]], sp["gSize"], C.size)
+            job = dict(spec=sp, C=C, data=ctx.to_device(sp["data"]), vis=vis,
+                       grid=ctx.zeros((sp["gSize"], sp["gSize"]), np.complex64),
+                       out=ctx.zeros(sp["nVis"], np.complex64), wCellSize=wcs,
+                       plan=Plan(ctx, vis, **tuning))
+            del du, dv, dw
+            self.jobs.append(job)
+        self.batch = Batch(ctx, [(j["plan"], j["data"], j["C"], j["grid"], j["out"])
+                                 for j in self.jobs])
+        self.points = int(sum(sp["nVis"] * sp["sSize"] ** 2 for sp in specs))
+
+    def zero_grids(self):
+        for j in self.jobs:
+            j["grid"].zero()
+
+    def run_plain(self):
+        for j in self.jobs:
+            grid_plain(self.ctx, j["vis"], j["data"], j["C"], j["grid"])
+        for j in self.jobs:
+            degrid_plain(self.ctx, j["vis"], j["grid"], j["C"], j["out"])
+
+    def run_tiled(self, rebuild: bool = True):
+        for j in self.jobs:
+            if rebuild:
+                j["plan"].build()
+            j["plan"].grid(j["data"], j["C"], j["grid"])
+            j["plan"].degrid(j["grid"], j["C"], j["out"])
+
+    def run_batched(self, graph: bool = True, rebuild: bool = True):
+        ops = (1 if rebuild else 0) | 2 | 4
+        if graph:
+            self.batch.run_graph(ops)
+        else:
+            if rebuild:
+                self.batch.rebuild()
+            self.batch.grid()
+            self.batch.degrid()

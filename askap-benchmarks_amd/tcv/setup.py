@@ -177,3 +177,53 @@ def current_problem(runType: int) -> Problem:
     return Problem(f"current-rt{runType}", u, v, w, wn, np.ones(u.size, np.complex64), C,
                    p["gSize"], p["uvCellSize"], p["wCellSize"], p["wSize"], p["overSample"],
                    p["support"], sSize, cOffset0, meta=dict(runType=runType))
+
+
+F0 = 1.4e9 / 2.998e8
+
+
+def attic_support(baseline: float, cellSize: float, freq0: float = F0) -> int:
+    """attic/tConvolveMPI/Benchmark.cc:212-213."""
+    return int(1.5 * np.sqrt(abs(baseline) * cellSize * freq0) / cellSize)
+
+
+def tightest_cell(baseline: float, gSize: int, freq0: float = F0) -> float:
+    """Smallest cellSize (wavelengths) for which every footprint of a
+    visibility with |u|, |v| <= baseline stays on the grid: the pixel radius R
+    plus the full kernel width (rows start at iv) must fit in gSize/2."""
+    # R + 2*support + 2 <= gSize/2 with R = f*B/c, support ~ 1.5*sqrt(f*B/c)
+    fb = freq0 * baseline
+    y = (-3.0 + np.sqrt(9.0 + 4.0 * (gSize / 2 - 2))) / 2.0   # y = sqrt(f*B/c)
+    c = fb / (y * y)
+    for _ in range(10000):
+        sup = attic_support(baseline, c, freq0)
+        R = int(np.ceil(fb / c)) + 1
+        if R + 2 * sup + 2 <= gSize // 2:
+            return float(c)
+        c *= 1.002
+    raise ValueError("no cell size fits")
+
+
+def config3(nJobs: int = 64, seed: int = 3, vis_min: float = 1e4, vis_max: float = 2e6,
+            grids=(1024, 2048, 4096), wsizes=(33, 65, 129)):
+    """The uneven batch: per job log-uniform visibility count, grid size and
+    wSize drawn from the sets, baseline uniform integer in [1000, 6000] m, u, v, w
+    uniform in [-B, B], complex Gaussian data, one channel at 1.4 GHz, cellSize
+    at the tightest fit (support from the reference formula).  C(w) tables are
+    generated on the device by the consumer (tcv.attic_initC_device)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    jobs = []
+    for j in range(nJobs):
+        n = int(np.exp(rng.uniform(np.log(vis_min), np.log(vis_max))))
+        g = int(rng.choice(grids))
+        ws = int(rng.choice(wsizes))
+        B = float(rng.integers(1000, 6001))   # integer metres, as initC takes them
+        c = tightest_cell(B, g)
+        u = rng.uniform(-B, B, n)
+        v = rng.uniform(-B, B, n)
+        w = rng.uniform(-B, B, n)
+        data = ((rng.standard_normal(n) + 1j * rng.standard_normal(n)) * np.sqrt(0.5)).astype(np.complex64)
+        sup = attic_support(B, c)
+        jobs.append(dict(job=j, nVis=n, gSize=g, wSize=ws, baseline=B, cellSize=c, support=sup,
+                         sSize=2 * sup + 1, u=u, v=v, w=w, data=data, freq=np.array([F0])))
+    return jobs

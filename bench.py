@@ -62,6 +62,50 @@ def cpu_baseline(prob, iu, iv, co, seconds_target=20.0):
     }
 
 
+def bench_batch(ctx, args):
+    """The uneven 64-job batch: the same jobs run as one persistent
+    hipGraph-captured launch (batched), as one tiled launch per job, and with
+    the plain one-workgroup-per-visibility kernels, one launch per job.  Each
+    mode re-bins (where it bins at all), grids and degrids every job."""
+    from tcv import setup as S
+    from tcv.gridder import BatchJobs
+    t0 = time.time()
+    specs = S.config3(nJobs=args.batch_jobs)
+    bj = BatchJobs(ctx, specs)
+    log(f"batch: {len(specs)} jobs, {sum(s['nVis'] for s in specs)} visibilities, "
+        f"{bj.points / 1e9:.1f} G points per pass, set up in {time.time() - t0:.1f} s")
+    pts = 2 * bj.points
+
+    def timed(fn, reps):
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(reps):
+            bj.zero_grids()
+            fn()
+        return ctx.timer_stop() / reps
+
+    timed(lambda: bj.run_batched(graph=True), 1)          # capture + warm
+    t_batch = timed(lambda: bj.run_batched(graph=True), args.batch_steps)
+    timed(lambda: bj.run_tiled(), 1)
+    t_tiled = timed(lambda: bj.run_tiled(), args.batch_steps)
+    t_plain = timed(lambda: bj.run_plain(), 1)
+    out = {
+        "workload": "64 jobs: visibilities log-uniform [1e4, 2e6], grids {1024,2048,4096}^2, "
+                    "wSize {33,65,129}, baselines 1000-6000 m, complex64, tightest-fit cellSize",
+        "jobs": len(specs), "visibilities": int(sum(s["nVis"] for s in specs)),
+        "points_per_step": pts,
+        "batched": {"ms": t_batch, "value": pts / (t_batch * 1e-3)},
+        "tiled_per_job": {"ms": t_tiled, "value": pts / (t_tiled * 1e-3)},
+        "plain_per_job": {"ms": t_plain, "value": pts / (t_plain * 1e-3)},
+    }
+    out["speedup_batched_vs_plain"] = t_plain / t_batch
+    out["speedup_batched_vs_tiled"] = t_tiled / t_batch
+    log(f"batch: batched {t_batch:.1f} ms, tiled/job {t_tiled:.1f} ms, plain/job {t_plain:.1f} ms "
+        f"-> batched/plain {out['speedup_batched_vs_plain']:.2f}x")
+    del bj
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +120,9 @@ def main():
     ap.add_argument("--phases", type=int, default=0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-jobs", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: the gridding path here runs on a single "
@@ -163,6 +210,11 @@ def main():
         result["plain"] = {"value": plain, "grid_ms": t_gp, "degrid_ms": t_dp,
                            "speedup_tiled_vs_plain": value / plain}
         log(f"plain: grid {t_gp:.1f} ms, degrid {t_dp:.1f} ms, {plain / 1e9:.1f} G points/s")
+
+    if not args.no_batch:
+        del job
+        ctx.sync()
+        result["batch"] = bench_batch(ctx, args)
 
     if not args.no_cpu:
         import oracle as O

@@ -137,8 +137,9 @@ tcv_status tcv_init_offsets(tcv_ctx *ctx, const double *d_u, const double *d_v,
 
 typedef struct {
     int nVis;                 /* visibilities (nSamples * nChan) */
-    const int *d_iu;          /* grid column of the footprint centre */
-    const int *d_iv;          /* grid row of the footprint centre */
+    const int *d_iu;          /* centre column: columns iu-s/2 .. iu-s/2+s-1 */
+    const int *d_iv;          /* first grid row of the footprint: rows iv .. iv+s-1 (the
+                                 reference offsets only the column by the support) */
     const int *d_cOffset;     /* start of the visibility's C slice (complex units) */
     const int *d_wPlane;      /* w-plane per visibility; NULL => uniform width */
     int gSize;                /* grid is gSize x gSize, row-major, index iu + gSize*iv */
@@ -206,6 +207,72 @@ tcv_status tcv_grid_tiled(tcv_ctx *ctx, tcv_plan *plan, const float *d_data,
  * partial sum over the tile is reduced with DPP and added to the output. */
 tcv_status tcv_degrid_tiled(tcv_ctx *ctx, tcv_plan *plan, const float *d_grid,
                             const float *d_C, float *d_data);
+
+/* ---- FP16 storage mode ------------------------------------------------------- */
+/* C(w) taps and visibilities stored as half (re, im) pairs, scaled by the
+ * max |component| of their w-plane (taps) or of the whole set (visibilities);
+ * accumulation stays FP32.  No reference counterpart: the FP16 storage mode
+ * is an extension of gridKernel / degridKernel that halves the C stream. */
+
+/* Converts the complex64 table d_C (nTaps taps) into d_Chalf (4 B per tap) and
+ * writes the per-plane scales to d_planeScale (nPlanes floats; host copy in
+ * planeScale when non-NULL).  Planes: taps [planeStart[k], planeStart[k+1])
+ * (host array, nPlanes + 1 entries), or, when planeStart is NULL, uniform
+ * planes of planeTaps taps (the attic layout, planeTaps = sSize^2 * os^2).
+ * Blocking. */
+tcv_status tcv_ctable_to_half(tcv_ctx *ctx, const float *d_C, long long nTaps, int nPlanes,
+                              const long long *planeStart, long long planeTaps,
+                              void *d_Chalf, float *d_planeScale, float *planeScale);
+/* Converts nVis complex64 visibilities to half pairs scaled by their max
+ * |component|, returned in *scale.  Blocking. */
+tcv_status tcv_vis_to_half(tcv_ctx *ctx, const float *d_data, int nVis, void *d_dhalf,
+                           float *scale);
+/* tcv_grid_tiled / tcv_degrid_tiled on FP16 storage.  planeTaps > 0: the
+ * w-plane of a visibility is cOffset / planeTaps (uniform layout); 0: the plan
+ * carries the w-plane from the geometry's d_wPlane. */
+tcv_status tcv_grid_tiled_half(tcv_ctx *ctx, tcv_plan *plan, const void *d_dhalf,
+                               float dataScale, const void *d_Chalf,
+                               const float *d_planeScale, long long planeTaps, float *d_grid);
+tcv_status tcv_degrid_tiled_half(tcv_ctx *ctx, tcv_plan *plan, const float *d_grid,
+                                 const void *d_Chalf, const float *d_planeScale,
+                                 long long planeTaps, float *d_data);
+
+/* ---- batching: many jobs in one persistent launch --------------------------- */
+/* No reference counterpart: the reference replicates whole processes per job
+ * (attic/tConvolveMPI/tConvolveMPI.cc:78-94); here independent jobs (channels,
+ * Taylor terms, snapshots) of uneven size share one launch over a device work
+ * queue. */
+typedef struct {
+    tcv_plan *plan;        /* built plan of the job; all plans of a batch share tile_w/tile_h */
+    const float *d_data;   /* complex64 visibilities to grid (may be NULL: degrid only) */
+    const float *d_C;      /* complex64 C(w) taps of the job */
+    float *d_grid;         /* complex64 gSize^2 grid (gridding accumulates into it) */
+    float *d_out;          /* complex64 degridding output (may be NULL: grid only) */
+} tcv_job;
+
+typedef struct tcv_batch tcv_batch;
+
+tcv_status tcv_batch_create(tcv_ctx *ctx, int nJobs, const tcv_job *jobs, tcv_batch **batch);
+tcv_status tcv_batch_destroy(tcv_ctx *ctx, tcv_batch *batch);
+/* Re-bins every job from the geometry its plan was last built with (queued,
+ * no host sync). */
+tcv_status tcv_batch_rebuild(tcv_ctx *ctx, tcv_batch *batch);
+/* One persistent launch over the work items of all jobs. */
+tcv_status tcv_batch_grid(tcv_ctx *ctx, tcv_batch *batch);
+tcv_status tcv_batch_degrid(tcv_ctx *ctx, tcv_batch *batch);
+/* ops: 1 = rebuild, 2 = grid, 4 = degrid (any sum).  The first call with a
+ * given ops value captures the work into a hipGraph; later calls replay it. */
+tcv_status tcv_batch_run_graph(tcv_ctx *ctx, tcv_batch *batch, int ops);
+
+/* attic C(w) (attic/tConvolveMPI/Benchmark.cc:206-270) generated on the
+ * device in FP64, normalised by the FP64 sum of |C| -- the reference sums in
+ * float, which stalls near 2^24 on large tables, so these tables differ from
+ * tcv_attic_initC's (bit-identical to the reference) by that normalisation
+ * factor and by last-ulp libm differences.  For synthetic batch and serving
+ * workloads.  Queued, no host sync. */
+tcv_status tcv_attic_initC_device(tcv_ctx *ctx, double freq0, double cellSize, int wSize,
+                                  int baseline, int overSample, int *support,
+                                  double *wCellSize, float *d_C);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,97 @@
+"""Batching: many uneven jobs (different grid sizes, kernel widths, w-plane
+counts and visibility counts) in one persistent launch, optionally captured
+in a hipGraph, checked job by job against the oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tcv
+from helpers import rel_err
+from tcv import setup as S
+from tcv.gridder import BatchJobs
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def small_specs(n=5, seed=11):
+    return S.config3(nJobs=n, seed=seed, vis_min=2e3, vis_max=2e4, grids=(512, 1024, 2048))
+
+
+def oracle_job(bj, j, grid_in=None):
+    sp = bj.jobs[j]["spec"]
+    vis = bj.jobs[j]["vis"]
+    iu, iv, co = vis.iu.download(), vis.iv.download(), vis.cOffset.download()
+    C = bj.jobs[j]["C"].download()
+    g = O.grid(sp["data"], iu, iv, co, C, sp["gSize"], sp["sSize"])
+    o = O.degrid(g if grid_in is None else grid_in, iu, iv, co, C, sp["sSize"])
+    return g, o
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_vs_oracle(ctx, graph):
+    bj = BatchJobs(ctx, small_specs())
+    bj.zero_grids()
+    bj.run_batched(graph=graph)
+    for j in range(len(bj.jobs)):
+        g = bj.jobs[j]["grid"].download()
+        gref, _ = oracle_job(bj, j)
+        assert rel_err(g, gref) <= TOL, j
+        # degrid compared on the GPU's own grid (same input both sides)
+        _, oref = oracle_job(bj, j, grid_in=g)
+        assert rel_err(bj.jobs[j]["out"].download(), oref) <= TOL, j
+
+
+def test_batch_modes_agree(ctx):
+    bj = BatchJobs(ctx, small_specs(6, seed=12))
+    res = {}
+    for mode in ("plain", "tiled", "batched"):
+        bj.zero_grids()
+        getattr(bj, "run_" + mode)()
+        res[mode] = [(j["grid"].download(), j["out"].download()) for j in bj.jobs]
+    for mode in ("tiled", "batched"):
+        for (g, o), (gp, op) in zip(res[mode], res["plain"]):
+            assert rel_err(g, gp) <= TOL
+            assert rel_err(o, op) <= TOL
+
+
+def test_batch_graph_replays(ctx):
+    """Replays of the captured graph, with eager work and synchronisations in
+    between, each equal to the eager batched result."""
+    bj = BatchJobs(ctx, small_specs(4, seed=13))
+    bj.zero_grids()
+    bj.run_batched(graph=False)
+    eager = [(j["grid"].download(), j["out"].download()) for j in bj.jobs]
+    for rep in range(3):
+        bj.zero_grids()
+        bj.run_batched(graph=True)
+        ctx.sync()
+        got = [(j["grid"].download(), j["out"].download()) for j in bj.jobs]
+        for (g, o), (ge, oe) in zip(got, eager):
+            assert rel_err(g, ge) <= TOL, rep
+            assert rel_err(o, oe) <= TOL, rep
+        # unrelated eager work between replays
+        j0 = bj.jobs[0]
+        j0["plan"].grid(j0["data"], j0["C"], j0["grid"])
+        ctx.sync()
+
+
+def test_device_initC_matches_host_taps(ctx):
+    """Device-generated attic taps equal the host table up to the
+    normalisation factor (FP64 sum on the device vs the reference's float
+    running sum) and last-ulp libm differences."""
+    f0 = S.F0
+    sup, wcs, dC = tcv.attic_initC_device(ctx, f0, 5.0, 33, 2000)
+    s2, w2, hC = S.attic_initC(f0, 5.0, 33, 2000)
+    assert sup == s2 and wcs == w2
+    d = dC.download()
+    assert not d.imag.any()
+    # compare shapes after removing each table's own normalisation
+    ratio = d.real.astype(np.float64) / np.where(hC.real != 0, hC.real, 1)
+    k = np.abs(hC.real) > 1e-3 * np.abs(hC.real).max()
+    r = np.median(ratio[k])
+    assert np.abs(ratio[k] / r - 1).max() < 1e-5
+    # the device table is normalised to sum |C| = wSize * os^2 in FP64
+    assert np.abs(d.astype(np.complex128)).sum() == pytest.approx(33 * 64, rel=1e-6)

@@ -1,0 +1,90 @@
+"""FP16 storage mode (C(w) and visibilities as half pairs, per-w-plane
+max-abs scaling, FP32 accumulation): errors against the FP64 oracle and the
+FP32 path, on the config-1 workload and a complex-C variable-width job."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tcv
+from helpers import rel_err, small_problem
+from tcv import setup as S
+
+pytestmark = pytest.mark.gpu
+# FP16 keeps 11 significant bits: each stored value is within 2^-11 of itself
+# (relative), so the accumulated error stays near 1e-3 of the maximum.
+HALF_TOL = 2e-3
+
+
+def _half_run(ctx, vis, data, C, nPlanes, planeTaps, gSize, planeStart=None):
+    d_C, d_data = ctx.to_device(C), ctx.to_device(data)
+    Ch = tcv.HalfTable(ctx, d_C, nPlanes, planeTaps, planeStart)
+    dh = tcv.HalfVis(ctx, d_data)
+    plan = tcv.Plan(ctx, vis)
+    g = ctx.zeros((gSize, gSize), np.complex64)
+    plan.grid_half(dh, Ch, g)
+    out = ctx.empty(vis.nVis, np.complex64)
+    plan.degrid_half(g, Ch, out)
+    return g.download(), out.download(), Ch
+
+
+def test_half_small_complex_vs_fp64(ctx):
+    Sz, gSize, os_, planes = 65, 320, 8, 5
+    iu, iv, co, data, C = small_problem(61, 4000, gSize, Sz, os_=os_, nPlanes=planes)
+    vis = tcv.Visibilities(ctx, iu, iv, co, [Sz], gSize, C.size)
+    g, out, Ch = _half_run(ctx, vis, data, C, planes, Sz * Sz * os_ * os_, gSize)
+    g64 = O.grid_f64(data, iu, iv, co, C, gSize, Sz)
+    e = rel_err(g, g64)
+    print(f"fp16 grid err vs fp64 {e:.2e}")
+    assert e <= HALF_TOL
+    o64 = O.degrid_f64(g.astype(np.complex128), iu, iv, co, C, Sz)
+    eo = rel_err(out, o64)
+    print(f"fp16 degrid err vs fp64 {eo:.2e}")
+    assert eo <= HALF_TOL
+    # the scales are the per-plane max |component|
+    Cp = C.reshape(planes, -1)
+    assert np.allclose(Ch.host_scales, np.maximum(np.abs(Cp.real), np.abs(Cp.imag)).max(1))
+
+
+def test_half_variable_width_planes(ctx):
+    """Current-generation layout (per-plane widths, complex C, planes from
+    planeStart and the plan's w-plane)."""
+    b = tcv.CurrentBenchmark(ctx)
+    b.setRunType(1)
+    b.init()
+    iu, iv = b.samples.iu.download(), b.samples.iv.download()
+    wp, co = b.samples.wPlane.download(), b.samples.cOffset.download()
+    C = b.C.download()
+    n = 200000
+    sel = np.random.default_rng(3).choice(iu.size, n, replace=False)
+    data = ((np.random.default_rng(4).standard_normal(n) + 1j) * 0.5).astype(np.complex64)
+    vis = tcv.Visibilities(ctx, iu[sel], iv[sel], co[sel], b.sSize, b.gSize, C.size, wPlane=wp[sel])
+    ps = np.append(b.cOffset0.astype(np.int64), C.size)
+    g, out, _ = _half_run(ctx, vis, data, C, b.wSize, 0, b.gSize, planeStart=ps)
+    g64 = O.grid_f64(data, iu[sel], iv[sel], co[sel], C, b.gSize, b.sSize, wPlane=wp[sel])
+    assert rel_err(g, g64) <= HALF_TOL
+    o64 = O.degrid_f64(g.astype(np.complex128), iu[sel], iv[sel], co[sel], C, b.sSize, wPlane=wp[sel])
+    assert rel_err(out, o64) <= HALF_TOL
+
+
+def test_half_config1_errors_reported(ctx):
+    """Config 1 in FP16 storage: max and RMS error of the FP32 and FP16 paths
+    against the FP64 oracle."""
+    prob = S.config1()
+    job = tcv.GridJob(ctx, prob)
+    iu, iv, _, co = job.offsets_host()
+    planeTaps = 129 * 129 * 64
+    g32 = (job.grid("tiled"), job.grid_d.download())[1]
+    g64 = O.grid_f64(prob.data, iu, iv, co, prob.C, 4096, 129)
+    Ch = tcv.HalfTable(ctx, job.C, 33, planeTaps)
+    dh = tcv.HalfVis(ctx, job.data)
+    gh = ctx.zeros((4096, 4096), np.complex64)
+    job.plan.grid_half(dh, Ch, gh)
+    g16 = gh.download()
+    m = np.abs(g64).max()
+    e32 = np.abs(g32 - g64).max() / m
+    e16 = np.abs(g16 - g64).max() / m
+    r16 = np.sqrt(np.mean(np.abs(g16 - g64) ** 2)) / m
+    print(f"config1 grid vs fp64: fp32 max {e32:.2e}, fp16 max {e16:.2e} rms {r16:.2e}")
+    assert e32 <= 1e-5 and e16 <= HALF_TOL
