@@ -376,15 +376,21 @@ tcv_status tcv_range_errors(tcv_ctx *ctx, long long *count)
 
 // ---- plans -------------------------------------------------------------------
 
-static void auto_tuning(int maxS, int gSize, int nVis, tcv_tuning *t)
+// Defaults (a tuning field <= 0 means auto; xcd_split < 0 means auto):
+// tile width from the kernel width, 64-row tiles, 2048-entry work items, and
+// entries ordered by C-slice key in 32 phases (each phase touches 1/32 of the
+// table, so concurrently running workgroups share the C slices in MALL/L2).
+static void auto_tuning(int maxS, const tcv_geometry *g, tcv_tuning *t)
 {
     if (t->tile_w <= 0) t->tile_w = maxS >= 48 ? 64 : maxS >= 24 ? 32 : maxS >= 12 ? 16 : 8;
     if (t->tile_h <= 0) t->tile_h = 64;
     if (t->chunk <= 0) t->chunk = 2048;
-    if (t->slice_keys < 0) t->slice_keys = 0;
-    if (t->phases <= 0) t->phases = 1;
-    (void)gSize;
-    (void)nVis;
+    int minS = maxS;
+    for (int k = 0; k < g->nPlanes; k++) minS = std::min(minS, g->sSize[k]);
+    const long long slices = std::max<long long>(1, g->cSize / ((long long)minS * minS));
+    if (t->slice_keys <= 0) t->slice_keys = (int)std::min<long long>(2048, slices);
+    if (t->phases <= 0) t->phases = std::min(32, t->slice_keys);
+    if (t->xcd_split < 0) t->xcd_split = 0;
 }
 
 static void free_plan(tcv_plan *p)
@@ -406,11 +412,11 @@ tcv_status tcv_plan_create(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning
     if (g->gSize > 65535) return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tiled path needs gSize <= 65535");
     if (g->cSize * 8 >= 0x7fffffffLL)
         return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tiled path needs a C table below 2 GiB");
-    tcv_tuning t = tuning ? *tuning : tcv_tuning{0, 0, 0, 0, 0};
-    auto_tuning(maxS, g->gSize, g->nVis, &t);
+    tcv_tuning t = tuning ? *tuning : tcv_tuning{0, 0, 0, 0, 0, -1};
+    auto_tuning(maxS, g, &t);
     if (!tiled_config_supported(t.tile_w, t.tile_h))
         return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");
-    if (t.chunk < 1 || t.phases < 1 || t.slice_keys < 0)
+    if (t.chunk < 1 || t.phases < 1 || t.slice_keys < 1)
         return tcv_fail(ctx, TCV_ERR_INVALID, "bad tuning");
     TCV_HIP(ctx, hipSetDevice(ctx->device));
 
@@ -421,6 +427,7 @@ tcv_status tcv_plan_create(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning
     sh.chunk = t.chunk;
     sh.k2 = std::max(1, t.slice_keys);
     sh.phases = std::min(t.phases, sh.k2);
+    sh.xcd = (t.xcd_split > 0 && sh.phases % 8 == 0) ? 1 : 0;
     sh.nTX = (g->gSize + sh.tw - 1) / sh.tw;
     sh.nTY = (g->gSize + sh.th - 1) / sh.th;
     sh.nTiles = sh.nTX * sh.nTY;
@@ -451,7 +458,7 @@ tcv_status tcv_plan_create(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning
               alloc((void **)&p->cursor, sizeof(int) * (sh.nKeys + 1)) &&
               alloc((void **)&p->itemCount, sizeof(int) * (slots + 1)) &&
               alloc((void **)&p->itemStart, sizeof(int) * (slots + 1)) &&
-              alloc((void **)&p->meta, sizeof(int) * 8) &&
+              alloc((void **)&p->meta, sizeof(int) * 32) &&
               alloc((void **)&p->scanTmp, sizeof(int) * p->scanTmpLen);
     if (!ok) {
         (void)hipGetLastError();
@@ -519,8 +526,8 @@ tcv_status tcv_plan_info(tcv_ctx *ctx, tcv_plan *p, tcv_tuning *t, long long *en
                          long long *items, long long *outOfRange)
 {
     if (!ctx || !p) return TCV_ERR_INVALID;
-    if (t) *t = tcv_tuning{p->sh.tw, p->sh.th, p->sh.chunk, p->sh.k2, p->sh.phases};
-    int meta[4] = {0, 0, 0, 0};
+    if (t) *t = tcv_tuning{p->sh.tw, p->sh.th, p->sh.chunk, p->sh.k2, p->sh.phases, p->sh.xcd};
+    int meta[4] = {0, 0, 0, 0};   // items, queue head, entries, out of range
     TCV_HIP(ctx, hipMemcpyAsync(meta, p->meta, sizeof(meta), hipMemcpyDeviceToHost, ctx->stream));
     TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (items) *items = meta[0];

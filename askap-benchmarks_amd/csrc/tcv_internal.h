@@ -76,7 +76,7 @@ tcv_status launch_degrid_plain(tcv_ctx *ctx, const GeomDev &g,
 struct JobDesc {
     const int4 *items;
     const int4 *entries;
-    const int *meta;            // meta[0] = number of work items
+    int *meta;                  // meta[0] = number of work items, per-XCD queues at [8..24]
     const void *data;           // visibilities to grid (float2, or half2 in FP16 storage)
     const void *C;              // taps (float2, or half2)
     float2 *grid;
@@ -91,7 +91,7 @@ struct JobDesc {
 };
 
 struct PlanShape {
-    int tw, th, chunk, k2, phases;
+    int tw, th, chunk, k2, phases, xcd;
     int nTX, nTY, nTiles;
     int maxS;
     long long maxEntries;  // upper bound on visibility x tile pairs
@@ -111,7 +111,8 @@ struct tcv_plan {
     int *cursor = nullptr;        // nKeys
     int *itemCount = nullptr;     // phases * nTiles + 1
     int *itemStart = nullptr;     // phases * nTiles + 1
-    int *meta = nullptr;          // [0] items [1] queue head [2] entries [3] bad
+    int *meta = nullptr;          // 32 ints: [0] items [1] queue head [2] entries [3] bad,
+                                  // [8..15] per-XCD heads, [16..24] per-XCD item ranges
     int *scanTmp = nullptr;       // scratch for the multi-level scan
     long long scanTmpLen = 0;
     long long cSize = 0;

@@ -315,7 +315,7 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_apply(const int *__restrict__ 
 // ---- tiled path: plan kernels -------------------------------------------------
 
 struct PlanDevArgs {
-    int tw, th, nTX, k2, phases, chunk, nTiles;
+    int tw, th, nTX, k2, phases, chunk, nTiles, xcd;
     long long cSize;
 };
 
@@ -358,6 +358,74 @@ __global__ __launch_bounds__(256) void k_plan_bin(int pass, GeomDev g, PlanDevAr
     }
 }
 
+// The same binning with the (tile, key) histogram aggregated in LDS, for key
+// spaces that fit (nKeys <= kLdsKeys): each block takes kBinVPT visibilities
+// per thread, counts locally, and touches each global counter at most once
+// (pass 0: add the block's count; pass 1: reserve the block's range, then place
+// entries at range start + LDS rank).  Cuts the global atomics on hot tile
+// counters by the number of visibilities per block.
+constexpr int kBinVPT = 16;
+constexpr int kLdsKeys = 6144;   // 2 x 6144 ints = 48 KB of LDS in pass 1
+
+__global__ __launch_bounds__(256) void k_plan_bin_lds(int pass, GeomDev g, PlanDevArgs a,
+                                                      int *__restrict__ counts,
+                                                      int *__restrict__ cursor,
+                                                      int4 *__restrict__ entries,
+                                                      int *__restrict__ meta)
+{
+    extern __shared__ int lds[];
+    const int nKeys = a.nTiles * a.k2;
+    int *hist = lds;
+    int *rank = lds + nKeys;
+    for (int k = threadIdx.x; k < (pass ? 2 : 1) * nKeys; k += blockDim.x) lds[k] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * blockDim.x * kBinVPT;
+    int isBad = 0;
+    for (int r = 0; r < kBinVPT; ++r) {
+        const int d = base + r * blockDim.x + threadIdx.x;
+        if (d >= g.nVis) break;
+        const VisGeom vg = vis_geom(g, d);
+        if (!vg.ok) {
+            isBad++;
+            continue;
+        }
+        const int key2 = (int)(((long long)vg.c0 * a.k2) / a.cSize);
+        const int tx0 = vg.u0 / a.tw, tx1 = (vg.u0 + vg.s - 1) / a.tw;
+        const int ty0 = vg.v0 / a.th, ty1 = (vg.v0 + vg.s - 1) / a.th;
+        for (int ty = ty0; ty <= ty1; ++ty)
+            for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&hist[(ty * a.nTX + tx) * a.k2 + key2], 1);
+    }
+    __syncthreads();
+    if (pass == 0) {
+        for (int k = threadIdx.x; k < nKeys; k += blockDim.x)
+            if (hist[k]) atomicAdd(&counts[k], hist[k]);
+        // reduce the out-of-range count over the wave, one atomic per wave
+        int nb = isBad;
+        for (int o = 32; o > 0; o >>= 1) nb += __shfl_xor(nb, o, 64);
+        if ((threadIdx.x & 63) == 0 && nb) atomicAdd(&meta[3], nb);
+        return;
+    }
+    for (int k = threadIdx.x; k < nKeys; k += blockDim.x)
+        if (hist[k]) hist[k] = atomicAdd(&cursor[k], hist[k]);
+    __syncthreads();
+    for (int r = 0; r < kBinVPT; ++r) {
+        const int d = base + r * blockDim.x + threadIdx.x;
+        if (d >= g.nVis) break;
+        const VisGeom vg = vis_geom(g, d);
+        if (!vg.ok) continue;
+        const int key2 = (int)(((long long)vg.c0 * a.k2) / a.cSize);
+        const int tx0 = vg.u0 / a.tw, tx1 = (vg.u0 + vg.s - 1) / a.tw;
+        const int ty0 = vg.v0 / a.th, ty1 = (vg.v0 + vg.s - 1) / a.th;
+        const int plane = g.wPlane ? g.wPlane[d] : 0;
+        const int4 ent = make_int4(vg.u0 | (vg.v0 << 16), vg.c0, d, vg.s | (plane << 16));
+        for (int ty = ty0; ty <= ty1; ++ty)
+            for (int tx = tx0; tx <= tx1; ++tx) {
+                const int key = (ty * a.nTX + tx) * a.k2 + key2;
+                entries[hist[key] + atomicAdd(&rank[key], 1)] = ent;
+            }
+    }
+}
+
 // Work items: each (phase, tile) range of entries is cut into chunks of at most
 // `chunk` entries.  Items are ordered phase-major so concurrently running
 // workgroups read neighbouring C slices.  Item = {tile, first, count, exclusive}.
@@ -367,9 +435,12 @@ __global__ __launch_bounds__(256) void k_plan_items(int pass, PlanDevArgs a,
                                                     const int *__restrict__ itemStart,
                                                     int4 *__restrict__ items)
 {
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= a.phases * a.nTiles) return;
-    const int p = id / a.nTiles, t = id - p * a.nTiles;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= a.phases * a.nTiles) return;
+    const int p = gid / a.nTiles, t = gid - p * a.nTiles;
+    // XCD-major slot order: sub-range p goes to XCD p % 8, so each XCD's items
+    // are contiguous and its L2 only sees its share of a phase's C slices
+    const int id = a.xcd ? ((p & 7) * (a.phases >> 3) + (p >> 3)) * a.nTiles + t : gid;
     const int k0 = (int)((long long)p * a.k2 / a.phases);
     const int k1 = (int)((long long)(p + 1) * a.k2 / a.phases);
     const long long b0 = (long long)t * a.k2;
@@ -388,18 +459,27 @@ __global__ __launch_bounds__(256) void k_plan_items(int pass, PlanDevArgs a,
     }
 }
 
+// meta: [0] items [1] queue head [2] entries [3] out of range
+//       [8..15] per-XCD queue heads, [16..24] per-XCD item ranges (XCD-major plans)
 __global__ void k_plan_meta(const int *__restrict__ itemStart, int nItemSlots,
-                            const int *__restrict__ starts, long long nKeys,
+                            const int *__restrict__ starts, long long nKeys, int xcdSlots,
                             int *__restrict__ meta)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         meta[0] = itemStart[nItemSlots];
         meta[1] = 0;
         meta[2] = starts[nKeys];
+        for (int x = 0; x < 8; ++x) meta[8 + x] = 0;
+        for (int x = 0; x <= 8; ++x)
+            meta[16 + x] = xcdSlots ? itemStart[x * xcdSlots] : (x ? meta[0] : 0);
     }
 }
 
-__global__ void k_reset_queue(int *meta) { if (threadIdx.x == 0) meta[1] = 0; }
+__global__ void k_reset_queue(int *meta)
+{
+    if (threadIdx.x < 8) meta[8 + threadIdx.x] = 0;
+    if (threadIdx.x == 0) meta[1] = 0;
+}
 
 // ---- tap storage ---------------------------------------------------------------
 // FP32: complex64 taps (8 B).  FP16: (re, im) half pairs (4 B) scaled by the
@@ -457,10 +537,27 @@ struct ItemSource {
     int nJobs;
     int *queue;
 
+    int xcd;   // single job: items are XCD-major with per-XCD queues
+
     __device__ __forceinline__ int total() const
     {
         if constexpr (BATCH) return __builtin_amdgcn_readfirstlane(prefix[nJobs]);
         else return __builtin_amdgcn_readfirstlane(one.meta[0]);
+    }
+    // next work item (one lane calls this); >= total() when none is left
+    __device__ __forceinline__ int acquire() const
+    {
+        if (BATCH || !xcd) return atomicAdd(queue, 1);
+        // own XCD first (hardware XCC_ID register), then steal from the others
+        const int x0 = (int)(__builtin_amdgcn_s_getreg((3 << 11) | 20) & 7);
+        for (int k = 0; k < 8; ++k) {
+            const int q = (x0 + k) & 7;
+            const int b = one.meta[16 + q], e = one.meta[17 + q];
+            if (b >= e) continue;
+            const int i = atomicAdd(one.meta + 8 + q, 1);
+            if (b + i < e) return b + i;
+        }
+        return 0x7fffffff;
     }
     // item -> (job descriptor, local item index); wave-uniform
     __device__ __forceinline__ void locate(int item, JobDesc &d, int &local) const
@@ -502,7 +599,7 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled(ItemSource<BATCH> src)
     const int q0 = wave * R;   // first row group of this wave
 
     for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(src.queue, 1);
+        if (threadIdx.x == 0) s_item = src.acquire();
         __syncthreads();
         const int item = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
@@ -599,7 +696,7 @@ __global__ __launch_bounds__(NW * 64) void k_degrid_tiled(ItemSource<BATCH> src)
     const int nItems = src.total();
 
     for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(src.queue, 1);
+        if (threadIdx.x == 0) s_item = src.acquire();
         __syncthreads();
         const int item = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
@@ -871,26 +968,31 @@ static tcv_status scan_exclusive(tcv_ctx *ctx, const int *in, int *out, long lon
 tcv_status plan_build_kernels(tcv_ctx *ctx, tcv_plan *p, const GeomDev &g)
 {
     const PlanShape &sh = p->sh;
-    PlanDevArgs a{sh.tw, sh.th, sh.nTX, sh.k2, sh.phases, sh.chunk, sh.nTiles, g.cSize};
+    PlanDevArgs a{sh.tw, sh.th, sh.nTX, sh.k2, sh.phases, sh.chunk, sh.nTiles, sh.xcd, g.cSize};
     ProfScope ps(ctx, "plan");
     tcv_status st;
     if ((st = launch_fill_u32(ctx, p->counts, 0, (size_t)sh.nKeys + 1)) != TCV_OK) return st;
-    if ((st = launch_fill_u32(ctx, p->meta, 0, 8)) != TCV_OK) return st;
-    const int nb = (g.nVis + 255) / 256;
-    if (g.nVis > 0) {
-        hipLaunchKernelGGL(k_plan_bin, dim3(nb), dim3(256), 0, ctx->stream, 0, g, a, p->counts,
-                           p->cursor, p->entries, p->meta);
+    if ((st = launch_fill_u32(ctx, p->meta, 0, 32)) != TCV_OK) return st;
+    const bool lds = sh.nKeys <= kLdsKeys;
+    const int nb = lds ? (g.nVis + 256 * kBinVPT - 1) / (256 * kBinVPT) : (g.nVis + 255) / 256;
+    auto bin = [&](int pass) -> tcv_status {
+        if (g.nVis <= 0) return TCV_OK;
+        if (lds)
+            hipLaunchKernelGGL(k_plan_bin_lds, dim3(nb), dim3(256),
+                               (size_t)(pass ? 2 : 1) * sh.nKeys * sizeof(int), ctx->stream, pass,
+                               g, a, p->counts, p->cursor, p->entries, p->meta);
+        else
+            hipLaunchKernelGGL(k_plan_bin, dim3(nb), dim3(256), 0, ctx->stream, pass, g, a,
+                               p->counts, p->cursor, p->entries, p->meta);
         TCV_HIP(ctx, hipGetLastError());
-    }
+        return TCV_OK;
+    };
+    if ((st = bin(0)) != TCV_OK) return st;
     if ((st = scan_exclusive(ctx, p->counts, p->starts, sh.nKeys, p->scanTmp)) != TCV_OK)
         return st;
     TCV_HIP(ctx, hipMemcpyAsync(p->cursor, p->starts, sizeof(int) * sh.nKeys,
                                 hipMemcpyDeviceToDevice, ctx->stream));
-    if (g.nVis > 0) {
-        hipLaunchKernelGGL(k_plan_bin, dim3(nb), dim3(256), 0, ctx->stream, 1, g, a, p->counts,
-                           p->cursor, p->entries, p->meta);
-        TCV_HIP(ctx, hipGetLastError());
-    }
+    if ((st = bin(1)) != TCV_OK) return st;
     const int slots = sh.phases * sh.nTiles;
     hipLaunchKernelGGL(k_plan_items, dim3((slots + 255) / 256), dim3(256), 0, ctx->stream, 0, a,
                        p->starts, p->itemCount, (const int *)p->itemStart, p->items);
@@ -901,7 +1003,8 @@ tcv_status plan_build_kernels(tcv_ctx *ctx, tcv_plan *p, const GeomDev &g)
                        p->starts, p->itemCount, (const int *)p->itemStart, p->items);
     TCV_HIP(ctx, hipGetLastError());
     hipLaunchKernelGGL(k_plan_meta, dim3(1), dim3(64), 0, ctx->stream, (const int *)p->itemStart,
-                       slots, (const int *)p->starts, sh.nKeys, p->meta);
+                       slots, (const int *)p->starts, sh.nKeys,
+                       sh.xcd ? (sh.phases / 8) * sh.nTiles : 0, p->meta);
     TCV_HIP(ctx, hipGetLastError());
     return TCV_OK;
 }
@@ -1004,7 +1107,7 @@ tcv_status launch_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const void *data, const 
                              float2 *grid, const HalfStorage *half)
 {
     ItemSource<false> src{make_job_desc(p, data, C, grid, nullptr, half), nullptr, nullptr, 0,
-                          p->meta + 1};
+                          p->meta + 1, p->sh.xcd};
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, p->meta);
     TCV_HIP(ctx, hipGetLastError());
 #define X(a, b)                                                                        \
@@ -1020,7 +1123,7 @@ tcv_status launch_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *grid, co
                                float2 *out, const HalfStorage *half)
 {
     ItemSource<false> src{make_job_desc(p, nullptr, C, const_cast<float2 *>(grid), out, half),
-                          nullptr, nullptr, 0, p->meta + 1};
+                          nullptr, nullptr, 0, p->meta + 1, p->sh.xcd};
     tcv_status st = launch_fill_u32(ctx, out, 0, (size_t)p->nVis * 2);
     if (st != TCV_OK) return st;
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, p->meta);
@@ -1044,7 +1147,7 @@ tcv_status launch_batch(tcv_ctx *ctx, tcv_batch *b, bool degrid)
     hipLaunchKernelGGL(k_batch_prefix, dim3(1), dim3(64), 0, ctx->stream,
                        (const JobDesc *)b->d_jobs, b->nJobs, b->d_prefix, b->d_queue);
     TCV_HIP(ctx, hipGetLastError());
-    ItemSource<true> src{JobDesc{}, b->d_jobs, b->d_prefix, b->nJobs, b->d_queue};
+    ItemSource<true> src{JobDesc{}, b->d_jobs, b->d_prefix, b->nJobs, b->d_queue, 0};
     // batches run FP32 storage
 #define X(a, c)                                                                        \
     if (b->tw == a && b->th == c)                                                      \

@@ -39,7 +39,7 @@ class Job(ct.Structure):
 
 class Tuning(ct.Structure):
     _fields_ = [("tile_w", I), ("tile_h", I), ("chunk", I), ("slice_keys", I),
-                ("phases", I)]
+                ("phases", I), ("xcd_split", I)]
 
 
 # name -> (restype, argtypes)

@@ -200,10 +200,10 @@ class Plan:
     """Device-side binning of a job's visibilities by grid tile (tcv_plan)."""
 
     def __init__(self, ctx: Context, vis: Visibilities, tile_w=0, tile_h=0, chunk=0,
-                 slice_keys=0, phases=0):
+                 slice_keys=0, phases=0, xcd_split=-1):
         self.ctx = ctx
         self.vis = vis
-        t = Tuning(tile_w, tile_h, chunk, slice_keys, phases)
+        t = Tuning(tile_w, tile_h, chunk, slice_keys, phases, xcd_split)
         g = vis.geometry()
         h = P()
         ctx.check(lib().tcv_plan_create(ctx.h, ct.byref(g), ct.byref(t), ct.byref(h)),
@@ -221,7 +221,8 @@ class Plan:
         self.ctx.check(lib().tcv_plan_info(self.ctx.h, self.h, ct.byref(t), ct.byref(e),
                                            ct.byref(it), ct.byref(bad)), "plan_info")
         return dict(tile_w=t.tile_w, tile_h=t.tile_h, chunk=t.chunk, slice_keys=t.slice_keys,
-                    phases=t.phases, entries=e.value, items=it.value, out_of_range=bad.value)
+                    phases=t.phases, xcd_split=t.xcd_split, entries=e.value, items=it.value,
+                    out_of_range=bad.value)
 
     def grid(self, data: DeviceArray, C: DeviceArray, grid: DeviceArray):
         self.ctx.check(lib().tcv_grid_tiled(self.ctx.h, self.h, data.ptr, C.ptr, grid.ptr),

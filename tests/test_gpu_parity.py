@@ -141,7 +141,9 @@ def test_tiled_shapes_vs_oracle(ctx, tw, th):
     vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
     gref = O.grid(data, iu, iv, co, C, gSize, Sz)
     for tuning in (dict(tile_w=tw, tile_h=th),
-                   dict(tile_w=tw, tile_h=th, chunk=64, slice_keys=256, phases=4)):
+                   dict(tile_w=tw, tile_h=th, slice_keys=1, phases=1),
+                   dict(tile_w=tw, tile_h=th, chunk=64, slice_keys=256, phases=4),
+                   dict(tile_w=tw, tile_h=th, chunk=96, slice_keys=320, phases=16, xcd_split=1)):
         g = gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning)
         assert rel_err(g, gref) <= TOL, tuning
         out = gpu_degrid(ctx, vis, gref, C, "tiled", tuning)
