@@ -71,6 +71,7 @@ const char *tcv_status_string(tcv_status s)
     case TCV_ERR_BUDGET: return "HBM budget exceeded";
     case TCV_ERR_UNSUPPORTED: return "unsupported";
     case TCV_ERR_CANCELLED: return "cancelled";
+    case TCV_ERR_TIMEOUT: return "timed out";
     }
     return "unknown status";
 }
@@ -139,6 +140,14 @@ tcv_status tcv_malloc(tcv_ctx *ctx, size_t bytes, void **d_ptr)
         (void)hipGetLastError();
         return tcv_fail(ctx, TCV_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
     }
+    return TCV_OK;
+}
+
+tcv_status tcv_mem_info(tcv_ctx *ctx, size_t *freeBytes, size_t *totalBytes)
+{
+    if (!ctx || !freeBytes || !totalBytes) return TCV_ERR_INVALID;
+    TCV_HIP(ctx, hipSetDevice(ctx->device));
+    TCV_HIP(ctx, hipMemGetInfo(freeBytes, totalBytes));
     return TCV_OK;
 }
 
@@ -395,17 +404,17 @@ static void auto_tuning(int maxS, const tcv_geometry *g, tcv_tuning *t)
 
 static void free_plan(tcv_plan *p)
 {
-    for (void *q : {(void *)p->d_sSize, (void *)p->entries, (void *)p->items, (void *)p->counts,
-                    (void *)p->starts, (void *)p->cursor, (void *)p->itemCount,
-                    (void *)p->itemStart, (void *)p->meta, (void *)p->scanTmp})
-        if (q) (void)hipFree(q);
+    if (p->ownsBlock && p->block) (void)hipFree(p->block);
+    p->block = nullptr;
 }
 
-tcv_status tcv_plan_create(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tuning,
-                           tcv_plan **out)
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // extern "C"
+
+tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tuning, tcv_plan *p,
+                      size_t *bytes)
 {
-    if (!out) return TCV_ERR_INVALID;
-    *out = nullptr;
     int maxS = 0;
     tcv_status st = check_geometry(ctx, g, &maxS);
     if (st != TCV_OK) return st;
@@ -418,9 +427,6 @@ tcv_status tcv_plan_create(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning
         return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");
     if (t.chunk < 1 || t.phases < 1 || t.slice_keys < 1)
         return tcv_fail(ctx, TCV_ERR_INVALID, "bad tuning");
-    TCV_HIP(ctx, hipSetDevice(ctx->device));
-
-    tcv_plan *p = new tcv_plan();
     PlanShape &sh = p->sh;
     sh.tw = t.tile_w;
     sh.th = t.tile_h;
@@ -435,10 +441,8 @@ tcv_status tcv_plan_create(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning
     const long long perVis = (long long)((maxS + sh.tw - 2) / sh.tw + 1) * ((maxS + sh.th - 2) / sh.th + 1);
     sh.maxEntries = std::max<long long>(1, (long long)g->nVis * perVis);
     sh.nKeys = (long long)sh.nTiles * sh.k2;
-    if (sh.maxEntries >= 0x7fffffffLL || sh.nKeys >= 0x7fffffffLL) {
-        delete p;
+    if (sh.maxEntries >= 0x7fffffffLL || sh.nKeys >= 0x7fffffffLL)
         return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "plan too large for 32-bit entry indices");
-    }
     p->maxItems = sh.maxEntries / sh.chunk + (long long)sh.phases * sh.nTiles + 1;
     p->nVis = g->nVis;
     p->gSize = g->gSize;
@@ -447,25 +451,65 @@ tcv_status tcv_plan_create(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning
     p->sSizeHost.assign(g->sSize, g->sSize + g->nPlanes);
     const long long slots = (long long)sh.phases * sh.nTiles;
     p->scanTmpLen = std::max(scan_tmp_len(sh.nKeys), scan_tmp_len(slots));
-    auto alloc = [&](void **q, size_t bytes) {
-        return hipMalloc(q, std::max<size_t>(bytes, 256)) == hipSuccess;
+    size_t n = 0;
+    n += align256(sizeof(int) * g->nPlanes);
+    n += align256(sizeof(int4) * sh.maxEntries);
+    n += align256(sizeof(int4) * p->maxItems);
+    n += 3 * align256(sizeof(int) * (sh.nKeys + 1));
+    n += 2 * align256(sizeof(int) * (slots + 1));
+    n += align256(sizeof(int) * 32);
+    n += align256(sizeof(int) * p->scanTmpLen);
+    *bytes = n;
+    return TCV_OK;
+}
+
+void plan_bind(tcv_plan *p, char *block)
+{
+    const PlanShape &sh = p->sh;
+    const long long slots = (long long)sh.phases * sh.nTiles;
+    char *q = block;
+    auto take = [&](size_t bytes) {
+        char *r = q;
+        q += align256(bytes);
+        return r;
     };
-    bool ok = alloc((void **)&p->d_sSize, sizeof(int) * g->nPlanes) &&
-              alloc((void **)&p->entries, sizeof(int4) * sh.maxEntries) &&
-              alloc((void **)&p->items, sizeof(int4) * p->maxItems) &&
-              alloc((void **)&p->counts, sizeof(int) * (sh.nKeys + 1)) &&
-              alloc((void **)&p->starts, sizeof(int) * (sh.nKeys + 1)) &&
-              alloc((void **)&p->cursor, sizeof(int) * (sh.nKeys + 1)) &&
-              alloc((void **)&p->itemCount, sizeof(int) * (slots + 1)) &&
-              alloc((void **)&p->itemStart, sizeof(int) * (slots + 1)) &&
-              alloc((void **)&p->meta, sizeof(int) * 32) &&
-              alloc((void **)&p->scanTmp, sizeof(int) * p->scanTmpLen);
-    if (!ok) {
-        (void)hipGetLastError();
-        free_plan(p);
+    p->d_sSize = (int *)take(sizeof(int) * p->nPlanes);
+    p->entries = (int4 *)take(sizeof(int4) * sh.maxEntries);
+    p->items = (int4 *)take(sizeof(int4) * p->maxItems);
+    p->counts = (int *)take(sizeof(int) * (sh.nKeys + 1));
+    p->starts = (int *)take(sizeof(int) * (sh.nKeys + 1));
+    p->cursor = (int *)take(sizeof(int) * (sh.nKeys + 1));
+    p->itemCount = (int *)take(sizeof(int) * (slots + 1));
+    p->itemStart = (int *)take(sizeof(int) * (slots + 1));
+    p->meta = (int *)take(sizeof(int) * 32);
+    p->scanTmp = (int *)take(sizeof(int) * p->scanTmpLen);
+    p->block = block;
+}
+
+extern "C" {
+
+tcv_status tcv_plan_create(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tuning,
+                           tcv_plan **out)
+{
+    if (!out) return TCV_ERR_INVALID;
+    *out = nullptr;
+    if (!ctx) return TCV_ERR_INVALID;
+    TCV_HIP(ctx, hipSetDevice(ctx->device));
+    tcv_plan *p = new tcv_plan();
+    size_t bytes = 0;
+    tcv_status st = plan_setup(ctx, g, tuning, p, &bytes);
+    if (st != TCV_OK) {
         delete p;
-        return tcv_fail(ctx, TCV_ERR_NOMEM, "plan allocation failed");
+        return st;
     }
+    void *block = nullptr;
+    if (hipMalloc(&block, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        delete p;
+        return tcv_fail(ctx, TCV_ERR_NOMEM, "plan allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    plan_bind(p, (char *)block);
+    p->ownsBlock = true;
     if (hipMemcpy(p->d_sSize, g->sSize, sizeof(int) * g->nPlanes, hipMemcpyHostToDevice) != hipSuccess) {
         free_plan(p);
         delete p;

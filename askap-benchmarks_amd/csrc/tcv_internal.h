@@ -118,7 +118,15 @@ struct tcv_plan {
     long long cSize = 0;
     GeomDev geom{};               // geometry of the last build (for batch rebuilds)
     bool hasGeom = false;
+    void *block = nullptr;        // one device block holding every buffer above
+    bool ownsBlock = false;       // block from hipMalloc (else from a caller's arena)
 };
+
+// Plan memory as one block: plan_setup fills the shape and returns the bytes
+// needed for geometry g; plan_bind carves the buffers out of such a block.
+tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tuning, tcv_plan *p,
+                      size_t *bytes);
+void plan_bind(tcv_plan *p, char *block);
 
 struct tcv_batch {
     int nJobs = 0;

@@ -22,7 +22,7 @@ PLL = ct.POINTER(ct.c_longlong)
 STATUS = {
     0: "TCV_OK", 1: "TCV_ERR_INVALID", 2: "TCV_ERR_RANGE", 3: "TCV_ERR_DEVICE",
     4: "TCV_ERR_NOMEM", 5: "TCV_ERR_BUDGET", 6: "TCV_ERR_UNSUPPORTED",
-    7: "TCV_ERR_CANCELLED",
+    7: "TCV_ERR_CANCELLED", 8: "TCV_ERR_TIMEOUT",
 }
 
 
@@ -35,6 +35,21 @@ class Geometry(ct.Structure):
 
 class Job(ct.Structure):
     _fields_ = [("plan", P), ("d_data", P), ("d_C", P), ("d_grid", P), ("d_out", P)]
+
+
+class ServerConfig(ct.Structure):
+    _fields_ = [("arenaBytes", ct.c_size_t), ("maxInflight", I), ("maxBatch", I),
+                ("tile_w", I), ("tile_h", I), ("chunk", I), ("slice_keys", I), ("phases", I),
+                ("autoRelease", I)]
+
+
+class Request(ct.Structure):
+    _fields_ = [("op", I), ("geom", Geometry), ("d_data", P), ("d_C", P), ("d_grid", P)]
+
+
+class Result(ct.Structure):
+    _fields_ = [("submitted", D), ("admitted", D), ("launched", D), ("completed", D),
+                ("status", I), ("d_result", P), ("points", LL)]
 
 
 class Tuning(ct.Structure):
@@ -87,6 +102,16 @@ SIGS = {
     "tcv_batch_degrid": (I, [P, P]),
     "tcv_batch_run_graph": (I, [P, P, I]),
     "tcv_attic_initC_device": (I, [P, D, D, I, I, I, PI, PD, P]),
+    "tcv_mem_info": (I, [P, ct.POINTER(ct.c_size_t), ct.POINTER(ct.c_size_t)]),
+    "tcv_server_create": (I, [P, ct.POINTER(ServerConfig), ct.POINTER(P)]),
+    "tcv_server_destroy": (I, [P]),
+    "tcv_server_submit": (I, [P, ct.POINTER(Request), PLL]),
+    "tcv_server_cancel": (I, [P, LL]),
+    "tcv_server_wait": (I, [P, LL, D, ct.POINTER(Result)]),
+    "tcv_server_release": (I, [P, LL]),
+    "tcv_server_stats": (I, [P, PLL, PLL, ct.POINTER(ct.c_size_t), ct.POINTER(ct.c_size_t)]),
+    "tcv_server_clock": (D, [P]),
+    "tcv_server_last_error": (ct.c_char_p, [P]),
 }
 
 _lib = None

@@ -5,7 +5,7 @@ import ctypes as ct
 
 import numpy as np
 
-from ._lib import STATUS, Geometry, Job, Tuning, lib
+from ._lib import STATUS, Geometry, Job, Request, Result, ServerConfig, Tuning, lib
 
 P = ct.c_void_p
 
@@ -98,6 +98,11 @@ class Context:
 
     def reset_stats(self):
         self.check(lib().tcv_reset_stats(self.h), "reset_stats")
+
+    def mem_info(self):
+        f, t = ct.c_size_t(), ct.c_size_t()
+        self.check(lib().tcv_mem_info(self.h, ct.byref(f), ct.byref(t)), "mem_info")
+        return f.value, t.value
 
     def range_errors(self) -> int:
         c = ct.c_longlong()
@@ -340,3 +345,76 @@ def attic_initC_device(ctx: Context, freq0: float, cellSize: float, wSize: int, 
                                            ct.byref(sup), ct.byref(wcs), C.ptr),
               "attic_initC_device")
     return sup.value, wcs.value, C
+
+
+class Server:
+    """Continuous-batching request loop on the device (tcv_server).
+
+    Requests reference resident inputs (a Visibilities geometry, C(w), and the
+    data to grid or the grid to degrid); the server's HBM arena holds each
+    request's plan and result."""
+
+    GRID, DEGRID = 0, 1
+
+    def __init__(self, ctx: Context, arena_bytes: int, max_inflight: int = 2, max_batch: int = 0,
+                 auto_release: bool = False, tile_w=0, tile_h=0, chunk=0, slice_keys=0, phases=0):
+        self.ctx = ctx
+        cfg = ServerConfig(int(arena_bytes), max_inflight, max_batch, tile_w, tile_h, chunk,
+                           slice_keys, phases, int(auto_release))
+        h = P()
+        ctx.check(lib().tcv_server_create(ctx.h, ct.byref(cfg), ct.byref(h)), "server_create")
+        self.h = h
+        self._keep = {}
+
+    def _check(self, st, what):
+        if st:
+            raise TcvError(st, f"{what}: {lib().tcv_server_last_error(self.h).decode()}")
+
+    def submit(self, op: int, vis: Visibilities, C: DeviceArray, data: DeviceArray = None,
+               grid: DeviceArray = None) -> int:
+        req = Request(op, vis.geometry(), data.ptr if data is not None else None, C.ptr,
+                      grid.ptr if grid is not None else None)
+        rid = ct.c_longlong()
+        self._check(lib().tcv_server_submit(self.h, ct.byref(req), ct.byref(rid)), "server_submit")
+        self._keep[rid.value] = vis   # the geometry's device arrays must outlive the request
+        return rid.value
+
+    def cancel(self, rid: int):
+        self._check(lib().tcv_server_cancel(self.h, rid), "server_cancel")
+
+    def wait(self, rid: int, timeout: float = 120.0) -> dict:
+        r = Result()
+        self._check(lib().tcv_server_wait(self.h, rid, timeout, ct.byref(r)), "server_wait")
+        return dict(submitted=r.submitted, admitted=r.admitted, launched=r.launched,
+                    completed=r.completed, status=r.status, d_result=r.d_result, points=r.points)
+
+    def download(self, res: dict, shape, dtype=np.complex64) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        self.ctx.check(lib().tcv_download(self.ctx.h, _p(out), res["d_result"], out.nbytes),
+                       "server result download")
+        return out
+
+    def release(self, rid: int):
+        self._check(lib().tcv_server_release(self.h, rid), "server_release")
+        self._keep.pop(rid, None)
+
+    def stats(self) -> dict:
+        q, r = ct.c_longlong(), ct.c_longlong()
+        u, c = ct.c_size_t(), ct.c_size_t()
+        self._check(lib().tcv_server_stats(self.h, ct.byref(q), ct.byref(r), ct.byref(u),
+                                           ct.byref(c)), "server_stats")
+        return dict(queued=q.value, running=r.value, arena_used=u.value, arena_capacity=c.value)
+
+    def clock(self) -> float:
+        return lib().tcv_server_clock(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tcv_server_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -34,7 +34,8 @@ typedef enum {
     TCV_ERR_NOMEM = 4,        /* device allocation failed */
     TCV_ERR_BUDGET = 5,       /* admission refused: HBM budget exceeded */
     TCV_ERR_UNSUPPORTED = 6,  /* shape outside what this build handles */
-    TCV_ERR_CANCELLED = 7     /* request cancelled or timed out */
+    TCV_ERR_CANCELLED = 7,    /* request cancelled */
+    TCV_ERR_TIMEOUT = 8       /* wait timed out (the request is still pending) */
 } tcv_status;
 
 typedef struct tcv_ctx tcv_ctx;
@@ -53,6 +54,8 @@ tcv_status tcv_synchronize(tcv_ctx *ctx);
 /* ---- device memory -------------------------------------------------------- */
 
 tcv_status tcv_malloc(tcv_ctx *ctx, size_t bytes, void **d_ptr);
+/* Free and total device memory in bytes. */
+tcv_status tcv_mem_info(tcv_ctx *ctx, size_t *freeBytes, size_t *totalBytes);
 tcv_status tcv_free(tcv_ctx *ctx, void *d_ptr);
 /* Blocking copies (they synchronise the context stream first). */
 tcv_status tcv_upload(tcv_ctx *ctx, void *d_dst, const void *src, size_t bytes);
@@ -276,6 +279,57 @@ tcv_status tcv_batch_run_graph(tcv_ctx *ctx, tcv_batch *batch, int ops);
 tcv_status tcv_attic_initC_device(tcv_ctx *ctx, double freq0, double cellSize, int wSize,
                                   int baseline, int overSample, int *support,
                                   double *wCellSize, float *d_C);
+
+/* ---- serving: continuous batching of requests --------------------------------- */
+/* No reference counterpart (the reference has no request loop; the nearest is
+ * the device-resident major cycle of current/tMajorACC/tMajorACC.cc:977-1043).
+ * A scheduler thread admits requests first-in first-out as the HBM arena (the
+ * budget) can hold their buffers, batches every admitted request into one
+ * persistent launch on a free stream slot, and overlaps up to maxInflight
+ * batches on the device.  C tables, visibilities and grids to degrid stay
+ * resident with the caller; the arena holds each request's plan and its result
+ * (a new grid, or degridded visibilities). */
+typedef struct tcv_server tcv_server;
+
+typedef struct {
+    size_t arenaBytes;   /* HBM budget for request buffers, allocated once */
+    int maxInflight;     /* batches concurrently on the device (1..8 streams) */
+    int maxBatch;        /* requests per batch (0 = all admitted) */
+    int tile_w, tile_h, chunk, slice_keys, phases;   /* plan tuning (0 = auto) */
+    int autoRelease;     /* 1: free each result when its request completes */
+} tcv_server_config;
+
+typedef struct {
+    int op;                /* 0 = grid into a new zeroed grid, 1 = degrid d_grid */
+    tcv_geometry geom;     /* resident index arrays (geom.sSize is copied) */
+    const float *d_data;   /* op 0: complex64 visibilities */
+    const float *d_C;      /* complex64 C(w) taps */
+    const float *d_grid;   /* op 1: complex64 gSize^2 grid */
+} tcv_request;
+
+typedef struct {
+    double submitted, admitted, launched, completed;   /* seconds, server clock */
+    tcv_status status;     /* TCV_OK, TCV_ERR_CANCELLED or the failure */
+    const float *d_result; /* op 0: gSize^2 grid; op 1: nVis visibilities (NULL after release) */
+    long long points;      /* nVis * sSize^2 (largest width when widths vary) */
+} tcv_result;
+
+tcv_status tcv_server_create(tcv_ctx *ctx, const tcv_server_config *cfg, tcv_server **srv);
+/* Stops the scheduler after the batches on the device finish; frees the arena. */
+tcv_status tcv_server_destroy(tcv_server *srv);
+/* Thread-safe.  TCV_ERR_BUDGET when the request can never fit the budget. */
+tcv_status tcv_server_submit(tcv_server *srv, const tcv_request *req, long long *id);
+/* Cancels a request that is still queued (TCV_ERR_UNSUPPORTED once admitted). */
+tcv_status tcv_server_cancel(tcv_server *srv, long long id);
+/* Waits until the request completed, failed or was cancelled, at most
+ * timeout_s seconds (TCV_ERR_TIMEOUT, request still pending). */
+tcv_status tcv_server_wait(tcv_server *srv, long long id, double timeout_s, tcv_result *res);
+/* Frees a finished request's result buffer and forgets the request. */
+tcv_status tcv_server_release(tcv_server *srv, long long id);
+tcv_status tcv_server_stats(tcv_server *srv, long long *queued, long long *running,
+                            size_t *arenaUsed, size_t *arenaCapacity);
+double tcv_server_clock(tcv_server *srv);
+const char *tcv_server_last_error(tcv_server *srv);
 
 #ifdef __cplusplus
 }
