@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdint>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -182,6 +183,15 @@ tcv_status tcv_zero(tcv_ctx *ctx, void *d_ptr, size_t bytes)
 {
     if (!ctx || (bytes && !d_ptr) || (bytes % 4)) return TCV_ERR_INVALID;
     return launch_fill_u32(ctx, d_ptr, 0u, bytes / 4);
+}
+
+tcv_status tcv_copy(tcv_ctx *ctx, void *d_dst, const void *d_src, size_t bytes)
+{
+    if (!ctx) return TCV_ERR_INVALID;
+    if ((bytes % 16) || (bytes && (!d_dst || !d_src)) || ((uintptr_t)d_dst % 16) ||
+        ((uintptr_t)d_src % 16))
+        return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_copy: pointers and size must be 16-byte multiples");
+    return launch_copy(ctx, d_dst, d_src, bytes / 16);
 }
 
 tcv_status tcv_timer_start(tcv_ctx *ctx)
@@ -386,9 +396,21 @@ tcv_status tcv_range_errors(tcv_ctx *ctx, long long *count)
 // ---- plans -------------------------------------------------------------------
 
 // Defaults (a tuning field <= 0 means auto; xcd_split < 0 means auto):
-// tile width from the kernel width, 64-row tiles, 2048-entry work items, and
-// entries ordered by C-slice key in 32 phases (each phase touches 1/32 of the
-// table, so concurrently running workgroups share the C slices in MALL/L2).
+// tile width from the kernel width, 64-row tiles, 2048-entry work items, up to
+// 8192 C-slice keys, entries ordered by key in phases of about 4 MB of taps
+// (concurrently running workgroups share the phase's slices in L2), and, with
+// 8 or more phases, work items ordered XCD-major with one queue per XCD so
+// each XCD walks its own phases.
+// C-slice phases: a power of two giving about 4 MB of taps per phase (one XCD's
+// L2), so that the slices one phase touches stay cached while its tiles run.
+static int default_phases(long long cSize, int sliceKeys)
+{
+    const long long target = std::max<long long>(1, cSize * 8 / (4LL << 20));
+    int ph = 1;
+    while (ph * 2LL <= target && ph * 2 <= sliceKeys) ph *= 2;
+    return ph;
+}
+
 static void auto_tuning(int maxS, const tcv_geometry *g, tcv_tuning *t)
 {
     if (t->tile_w <= 0) t->tile_w = maxS >= 48 ? 64 : maxS >= 24 ? 32 : maxS >= 12 ? 16 : 8;
@@ -397,9 +419,9 @@ static void auto_tuning(int maxS, const tcv_geometry *g, tcv_tuning *t)
     int minS = maxS;
     for (int k = 0; k < g->nPlanes; k++) minS = std::min(minS, g->sSize[k]);
     const long long slices = std::max<long long>(1, g->cSize / ((long long)minS * minS));
-    if (t->slice_keys <= 0) t->slice_keys = (int)std::min<long long>(2048, slices);
-    if (t->phases <= 0) t->phases = std::min(32, t->slice_keys);
-    if (t->xcd_split < 0) t->xcd_split = 0;
+    if (t->slice_keys <= 0) t->slice_keys = (int)std::min<long long>(8192, slices);
+    if (t->phases <= 0) t->phases = default_phases(g->cSize, t->slice_keys);
+    if (t->xcd_split < 0) t->xcd_split = (t->phases >= 8 && t->phases % 8 == 0) ? 1 : 0;
 }
 
 static void free_plan(tcv_plan *p)
@@ -422,6 +444,10 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
     if (g->cSize * 8 >= 0x7fffffffLL)
         return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tiled path needs a C table below 2 GiB");
     tcv_tuning t = tuning ? *tuning : tcv_tuning{0, 0, 0, 0, 0, -1};
+    if (!tuning && ctx) {
+        auto it = ctx->tuned.find(tuning_class(g));
+        if (it != ctx->tuned.end()) t = it->second;
+    }
     auto_tuning(maxS, g, &t);
     if (!tiled_config_supported(t.tile_w, t.tile_h))
         return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");
@@ -805,6 +831,93 @@ tcv_status tcv_attic_initC_device(tcv_ctx *ctx, double freq0, double cellSize, i
     double *d_sum = reinterpret_cast<double *>(ctx->d_counters + 4);
     return launch_attic_taps(ctx, freq0, cellSize, wSize, 2 * *support + 1, overSample, *wCellSize,
                              (float2 *)d_C, d_sum);
+}
+
+// ---- autotuner ---------------------------------------------------------------------
+
+}  // extern "C"
+
+std::string tuning_class(const tcv_geometry *g)
+{
+    int maxS = 0;
+    for (int k = 0; k < g->nPlanes; k++) maxS = std::max(maxS, g->sSize[k]);
+    int lv = 0;
+    while ((1LL << (lv + 1)) <= std::max(1, g->nVis)) lv++;
+    int lc = 0;
+    while ((1LL << (lc + 1)) <= std::max<long long>(1, g->cSize)) lc++;
+    return std::to_string(g->gSize) + "/" + std::to_string(maxS) + "/" + std::to_string(lc) + "/" +
+           std::to_string(lv) + "/" + std::to_string(g->d_wPlane ? 1 : 0);
+}
+
+static std::vector<tcv_tuning> builtin_candidates(const tcv_geometry *g)
+{
+    int maxS = 0, minS = 1 << 30;
+    for (int k = 0; k < g->nPlanes; k++) {
+        maxS = std::max(maxS, g->sSize[k]);
+        minS = std::min(minS, g->sSize[k]);
+    }
+    const long long slices = std::max<long long>(1, g->cSize / ((long long)minS * minS));
+    const int k2 = (int)std::min<long long>(8192, slices);
+    const int ph = default_phases(g->cSize, k2);
+    std::vector<std::pair<int, int>> tiles;
+    if (maxS >= 48) tiles = {{64, 64}, {64, 32}};
+    else if (maxS >= 24) tiles = {{32, 64}, {32, 32}, {64, 32}};
+    else if (maxS >= 12) tiles = {{16, 64}, {16, 32}, {32, 32}};
+    else tiles = {{8, 64}, {8, 32}, {16, 32}};
+    std::vector<tcv_tuning> c;
+    for (auto &t : tiles)
+        for (int p : {ph / 2, ph, ph * 2}) {
+            if (p < 1 || p > k2) continue;
+            c.push_back(tcv_tuning{t.first, t.second, 2048, k2, p, p >= 8 ? 1 : 0});
+        }
+    c.push_back(tcv_tuning{tiles[0].first, tiles[0].second, 512, k2, ph, ph >= 8 ? 1 : 0});
+    c.push_back(tcv_tuning{tiles[0].first, tiles[0].second, 2048, 1, 1, 0});   // no ordering
+    return c;
+}
+
+extern "C" {
+
+tcv_status tcv_autotune(tcv_ctx *ctx, const tcv_geometry *g, const float *d_data, const float *d_C,
+                        float *d_grid, float *d_out, int op, const tcv_tuning *candidates,
+                        int nCandidates, int reps, tcv_tuning *best, double *bestMs)
+{
+    int maxS = 0;
+    tcv_status st = check_geometry(ctx, g, &maxS);
+    if (st != TCV_OK) return st;
+    if (op < 1 || op > 3 || reps < 1 || !best || !d_C || ((op & 1) && (!d_data || !d_grid)) ||
+        ((op & 2) && (!d_grid || !d_out)))
+        return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_autotune: bad argument");
+    std::vector<tcv_tuning> cand = candidates ? std::vector<tcv_tuning>(candidates, candidates + nCandidates)
+                                              : builtin_candidates(g);
+    double bestT = 1e300;
+    bool any = false;
+    for (const tcv_tuning &t : cand) {
+        tcv_plan *p = nullptr;
+        if (tcv_plan_create(ctx, g, &t, &p) != TCV_OK) continue;   // skip shapes that do not fit
+        auto once = [&]() -> tcv_status {
+            tcv_status s2 = TCV_OK;
+            if (op & 1) s2 = tcv_grid_tiled(ctx, p, d_data, d_C, d_grid);
+            if (s2 == TCV_OK && (op & 2)) s2 = tcv_degrid_tiled(ctx, p, d_grid, d_C, d_out);
+            return s2;
+        };
+        st = once();
+        double ms = 0;
+        if (st == TCV_OK) st = tcv_timer_start(ctx);
+        for (int r = 0; r < reps && st == TCV_OK; r++) st = once();
+        if (st == TCV_OK) st = tcv_timer_stop(ctx, &ms);
+        tcv_plan_destroy(ctx, p);
+        if (st != TCV_OK) return st;
+        ms /= reps;
+        if (ms < bestT) {
+            bestT = ms;
+            *best = t;
+            any = true;
+        }
+    }
+    if (!any) return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tcv_autotune: no candidate could be planned");
+    if (bestMs) *bestMs = bestT;
+    ctx->tuned[tuning_class(g)] = *best;
+    return TCV_OK;
 }
 
 }  // extern "C"

@@ -23,7 +23,12 @@ struct tcv_ctx {
     std::map<std::string, Stat> stats;
     // device counters: [0] visibilities skipped by the plain kernels (range)
     unsigned long long *d_counters = nullptr;
+    // autotuner results per workload class
+    std::map<std::string, tcv_tuning> tuned;
 };
+
+// Workload class of a geometry for the autotuner cache.
+std::string tuning_class(const tcv_geometry *g);
 
 tcv_status tcv_fail(tcv_ctx *ctx, tcv_status s, const std::string &msg);
 
@@ -57,6 +62,7 @@ struct GeomDev {
 
 // Launchers implemented in tcv_kernels.hip.  All enqueue on ctx->stream.
 tcv_status launch_fill_u32(tcv_ctx *ctx, void *p, unsigned value, size_t n32);
+tcv_status launch_copy(tcv_ctx *ctx, void *dst, const void *src, size_t n16);
 tcv_status launch_offsets(tcv_ctx *ctx, const double *u, const double *v,
                           const double *w, int nSamples, const double *d_freq,
                           int nChan, double cellSize, double wCellSize, int wSize,

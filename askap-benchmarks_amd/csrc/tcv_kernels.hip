@@ -63,6 +63,16 @@ __global__ void k_fill_u32(unsigned *p, unsigned v, size_t n)
     for (; i < n; i += stride) p[i] = v;
 }
 
+// Streaming device copy, 16 B per lane and iteration: a known-size HBM read and
+// write (the calibration kernel for hardware traffic counters).
+__global__ __launch_bounds__(256) void k_copy_u4(uint4 *__restrict__ dst,
+                                                  const uint4 *__restrict__ src, size_t n16)
+{
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+
 // ---- a6: FP64 index set-up ----------------------------------------------------
 // attic/tConvolveMPI/Benchmark.cc:293-322 and current/tConvolveMPI/Benchmark.cc:610-643.
 // Same expression order as the reference; contraction off so every product and
@@ -879,6 +889,17 @@ tcv_status launch_fill_u32(tcv_ctx *ctx, void *p, unsigned value, size_t n32)
     if (n32 == 0) return TCV_OK;
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_blocks(n32, 256)), dim3(256), 0, ctx->stream,
                        (unsigned *)p, value, n32);
+    TCV_HIP(ctx, hipGetLastError());
+    return TCV_OK;
+}
+
+tcv_status launch_copy(tcv_ctx *ctx, void *dst, const void *src, size_t n16)
+{
+    if (n16 == 0) return TCV_OK;
+    ProfScope ps(ctx, "copy");
+    const size_t blocks = std::min<size_t>((n16 + 255) / 256, (size_t)ctx->cus * 16);
+    hipLaunchKernelGGL(k_copy_u4, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                       (uint4 *)dst, (const uint4 *)src, n16);
     TCV_HIP(ctx, hipGetLastError());
     return TCV_OK;
 }

@@ -103,6 +103,9 @@ SIGS = {
     "tcv_batch_run_graph": (I, [P, P, I]),
     "tcv_attic_initC_device": (I, [P, D, D, I, I, I, PI, PD, P]),
     "tcv_mem_info": (I, [P, ct.POINTER(ct.c_size_t), ct.POINTER(ct.c_size_t)]),
+    "tcv_copy": (I, [P, P, P, ct.c_size_t]),
+    "tcv_autotune": (I, [P, ct.POINTER(Geometry), P, P, P, P, I, ct.POINTER(Tuning), I, I,
+                         ct.POINTER(Tuning), PD]),
     "tcv_server_create": (I, [P, ct.POINTER(ServerConfig), ct.POINTER(P)]),
     "tcv_server_destroy": (I, [P]),
     "tcv_server_submit": (I, [P, ct.POINTER(Request), PLL]),

@@ -140,6 +140,12 @@ class DeviceArray:
     def zero(self):
         self.ctx.check(lib().tcv_zero(self.ctx.h, self.ptr, self.nbytes), "zero")
 
+    def copy_from(self, src: "DeviceArray"):
+        """Device-to-device copy by the library's streaming kernel (tcv_copy)."""
+        if src.nbytes != self.nbytes:
+            raise ValueError("copy_from: size mismatch")
+        self.ctx.check(lib().tcv_copy(self.ctx.h, self.ptr, src.ptr, self.nbytes), "copy")
+
     def free(self):
         if getattr(self, "ptr", None) and self.ctx.h:
             lib().tcv_free(self.ctx.h, self.ptr)
@@ -201,6 +207,28 @@ def degrid_plain(ctx: Context, vis: Visibilities, grid: DeviceArray, C: DeviceAr
               "degrid_plain")
 
 
+def autotune(ctx: Context, vis: Visibilities, data: DeviceArray, C: DeviceArray,
+             grid: DeviceArray, out: DeviceArray, op: int = 3, candidates=None, reps: int = 2):
+    """Times candidate tunings of the tiled path on this job and returns the
+    fastest as a dict (cached in the context for plans made without tuning)."""
+    g = vis.geometry()
+    arr, n = None, 0
+    if candidates:
+        arr = (Tuning * len(candidates))(*[Tuning(c.get("tile_w", 0), c.get("tile_h", 0),
+                                                  c.get("chunk", 0), c.get("slice_keys", 0),
+                                                  c.get("phases", 0), c.get("xcd_split", -1))
+                                           for c in candidates])
+        n = len(candidates)
+    best = Tuning()
+    ms = ct.c_double()
+    ctx.check(lib().tcv_autotune(ctx.h, ct.byref(g), data.ptr if data is not None else None, C.ptr,
+                                 grid.ptr, out.ptr if out is not None else None, op, arr, n, reps,
+                                 ct.byref(best), ct.byref(ms)), "autotune")
+    return dict(tile_w=best.tile_w, tile_h=best.tile_h, chunk=best.chunk,
+                slice_keys=best.slice_keys, phases=best.phases, xcd_split=best.xcd_split,
+                ms=ms.value)
+
+
 class Plan:
     """Device-side binning of a job's visibilities by grid tile (tcv_plan)."""
 
@@ -208,11 +236,14 @@ class Plan:
                  slice_keys=0, phases=0, xcd_split=-1):
         self.ctx = ctx
         self.vis = vis
+        explicit = (tile_w, tile_h, chunk, slice_keys, phases, xcd_split) != (0, 0, 0, 0, 0, -1)
         t = Tuning(tile_w, tile_h, chunk, slice_keys, phases, xcd_split)
         g = vis.geometry()
         h = P()
-        ctx.check(lib().tcv_plan_create(ctx.h, ct.byref(g), ct.byref(t), ct.byref(h)),
-                  "plan_create")
+        # no explicit tuning: the library uses the context's autotuned choice for
+        # this workload class when there is one, else its built-in defaults
+        ctx.check(lib().tcv_plan_create(ctx.h, ct.byref(g), ct.byref(t) if explicit else None,
+                                        ct.byref(h)), "plan_create")
         self.h = h
 
     def build(self, vis: Visibilities | None = None):

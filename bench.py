@@ -42,6 +42,41 @@ def alg_bytes(points, nVis, gSize, op):
     return 8 * points + grid_bytes + 20 * nVis
 
 
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_grid_degrid_tiled_spectral.md")
+TUNING_KEYS = ("tile_w", "tile_h", "chunk", "slice_keys", "phases", "xcd_split")
+
+
+def measured_traffic(kernel, info):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel
+    calibration), when it was measured with the same tuning; else None."""
+    try:
+        lines = open(TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return None
+    rec = {}
+    for ln in lines:
+        if ln.startswith("traffic-record:"):
+            rec = json.loads(ln.split(":", 1)[1])
+    if not rec or any(rec.get("tuning", {}).get(k) != info.get(k) for k in TUNING_KEYS):
+        return None
+    return rec.get("bytes_per_launch", {}).get(kernel)
+
+
+def calibration_copy(ctx, nbytes=1 << 30, reps=2):
+    """Launches the streaming copy kernel on a buffer of known size so that a
+    counter run over this program also reads the counters' scale."""
+    import numpy as np
+    a = ctx.empty(nbytes // 4, np.uint32)
+    b = ctx.empty(nbytes // 4, np.uint32)
+    a.zero()
+    for _ in range(reps):
+        b.copy_from(a)
+    ctx.sync()
+    a.free()
+    b.free()
+
+
 def cpu_baseline(prob, iu, iv, co, seconds_target=20.0):
     import oracle as O
     cores = min(os.cpu_count() or 1, 16)
@@ -106,6 +141,82 @@ def bench_batch(ctx, args):
     return out
 
 
+def bench_serving(ctx, args, capacity):
+    """Poisson request streams against the continuous-batching server.
+
+    Requests are drawn from the 64 resident uneven-batch jobs (so sizes follow
+    that distribution): 50% grid into a new grid, 50% degrid the job's
+    resident grid.  Admission is by the HBM arena, sized so that resident data
+    plus the arena stay within a 200 GB budget.  Two streams: the specified
+    rate, and a declared rate at 60% of the measured batched capacity."""
+    from tcv import setup as S
+    from tcv.gridder import BatchJobs
+    specs = S.config3(nJobs=64)
+    bj = BatchJobs(ctx, specs)
+    bj.zero_grids()
+    bj.run_batched(graph=False)          # resident grids for the degrid requests
+    ctx.sync()
+    free, total = ctx.mem_info()
+    resident = total - free
+    budget = 200 * 10**9
+    arena = int(min(budget - resident, free - 16 * 2**30))
+    import tcv
+    srv = tcv.Server(ctx, arena, max_inflight=3, auto_release=True)
+    mean_pts = float(np.mean([sp["nVis"] * sp["sSize"] ** 2 for sp in specs]))
+
+    def stream(rate, n, seed):
+        rng = np.random.default_rng(seed)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+        ops = rng.integers(0, 2, n)
+        tmpl = rng.integers(0, len(bj.jobs), n)
+        t0 = srv.clock() + 0.05
+        ids = []
+        for i in range(n):
+            target = t0 + arrivals[i]
+            while True:
+                d = target - srv.clock()
+                if d <= 0:
+                    break
+                time.sleep(min(d, 0.002) if d > 0.0005 else 0)
+            j = bj.jobs[tmpl[i]]
+            if ops[i] == 0:
+                ids.append(srv.submit(0, j["vis"], j["C"], data=j["data"]))
+            else:
+                ids.append(srv.submit(1, j["vis"], j["C"], grid=j["grid"]))
+        res = []
+        for r in ids:
+            res.append(srv.wait(r, timeout=900))
+            srv.release(r)
+        bad = sum(r["status"] != 0 for r in res)
+        lat = np.array([r["completed"] - r["submitted"] for r in res]) * 1e3
+        wait = np.array([r["launched"] - r["submitted"] for r in res]) * 1e3
+        pts = float(sum(r["points"] for r in res))
+        span = max(r["completed"] for r in res) - min(r["submitted"] for r in res)
+        offered = rate * mean_pts
+        return {"rate_req_s": rate, "requests": n, "failed": bad,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+                "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)),
+                "queue_wait_p50_ms": float(np.percentile(wait, 50)),
+                "throughput": pts / span, "offered_points_per_s": offered,
+                "utilisation": offered / capacity}
+
+    out = {"workload": "Poisson requests over the 64 uneven jobs, 50% grid into new grids, 50% degrid "
+                       "resident grids, HBM arena admission, 3 overlapping batches",
+           "arena_bytes": arena, "resident_bytes": resident, "budget_bytes": budget,
+           "capacity_points_per_s": capacity, "mean_points_per_request": mean_pts}
+    out["specified"] = stream(200.0, args.serve_requests, 1)
+    log(f"serving @200/s: p50 {out['specified']['p50_ms']:.1f} ms p99 {out['specified']['p99_ms']:.1f} ms, "
+        f"utilisation {out['specified']['utilisation']:.2f}")
+    rate = 0.6 * capacity / mean_pts
+    out["declared_60pct"] = stream(rate, max(200, args.serve_requests // 2), 2)
+    log(f"serving @{rate:.0f}/s: p50 {out['declared_60pct']['p50_ms']:.1f} ms "
+        f"p99 {out['declared_60pct']['p99_ms']:.1f} ms")
+    srv.close()
+    del bj
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,11 +229,14 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--slice-keys", type=int, default=0)
     ap.add_argument("--phases", type=int, default=0)
+    ap.add_argument("--xcd-split", type=int, default=-1)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: the gridding path here runs on a single "
@@ -138,7 +252,7 @@ def main():
         f"C {prob.C.nbytes / 2**30:.2f} GiB")
     ctx = tcv.Context(0)
     tuning = dict(tile_w=args.tile_w, tile_h=args.tile_h, chunk=args.chunk,
-                  slice_keys=args.slice_keys, phases=args.phases)
+                  slice_keys=args.slice_keys, phases=args.phases, xcd_split=args.xcd_split)
     job = tcv.GridJob(ctx, prob, tuning=tuning)
     assert job.nBad == 0, f"{job.nBad} visibilities out of range"
     points = job.points
@@ -151,6 +265,7 @@ def main():
         job.grid("tiled")
         job.degrid("tiled")
 
+    calibration_copy(ctx)
     for _ in range(args.warmup):
         step()
     ctx.sync()
@@ -178,7 +293,8 @@ def main():
     bytes_l = alg_bytes(points, prob.nVis, prob.gSize, op)
     achieved = bytes_l / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, info),
+                "kernel": dom,
                 "avg_launch_ms": avg_ms, "alg_bytes_per_launch": bytes_l}
     per_kernel = {k: {"launches": v[0], "avg_ms": (v[1] / v[0]) if v[0] else None}
                   for k, v in stats.items()}
@@ -215,6 +331,8 @@ def main():
         del job
         ctx.sync()
         result["batch"] = bench_batch(ctx, args)
+        if not args.no_serve:
+            result["serving"] = bench_serving(ctx, args, result["batch"]["batched"]["value"])
 
     if not args.no_cpu:
         import oracle as O

@@ -63,6 +63,12 @@ tcv_status tcv_download(tcv_ctx *ctx, void *dst, const void *d_src, size_t bytes
 /* Zero fill by kernel (safe inside a captured graph). */
 tcv_status tcv_zero(tcv_ctx *ctx, void *d_ptr, size_t bytes);
 
+/* Device-to-device copy of `bytes` (a multiple of 16, both pointers 16-byte
+ * aligned) by a plain streaming kernel on the context stream.  No counterpart
+ * in the reference; its fixed traffic (bytes read + bytes written) calibrates
+ * hardware memory-traffic counters against the gridding kernels. */
+tcv_status tcv_copy(tcv_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
+
 /* ---- timing (hipEvents on the context stream) ----------------------------- */
 
 tcv_status tcv_timer_start(tcv_ctx *ctx);
@@ -191,7 +197,9 @@ typedef struct tcv_plan tcv_plan;
 
 /* Allocates a plan for jobs of this geometry class (device buffers sized from
  * the geometry's upper bounds) and builds it from g's index arrays.  tuning
- * may be NULL (auto).  Blocking. */
+ * may be NULL: the context's tcv_autotune choice for this workload class when
+ * there is one, else built-in defaults.  Zero fields of a given tuning are
+ * filled with the defaults.  Blocking. */
 tcv_status tcv_plan_create(tcv_ctx *ctx, const tcv_geometry *g,
                            const tcv_tuning *tuning, tcv_plan **plan);
 /* Re-bins the visibilities of g (same nVis/gSize/sSize class as at creation;
@@ -203,6 +211,21 @@ tcv_status tcv_plan_destroy(tcv_ctx *ctx, tcv_plan *plan);
 tcv_status tcv_plan_info(tcv_ctx *ctx, tcv_plan *plan, tcv_tuning *tuning,
                          long long *entries, long long *items,
                          long long *outOfRange);
+
+/* Autotuner: times candidate tunings of the tiled path on this job (hipEvents
+ * on the context stream, `reps` launches each after one warm-up) and returns
+ * the fastest in *best (and its average milliseconds in *bestMs).  op: 1 =
+ * gridding, 2 = degridding, 3 = both (summed).  candidates == NULL uses the
+ * built-in set: tile shapes suited to the kernel width, C-slice phase counts
+ * around the L2-sized target (about 4 MB of C per phase) with and without the
+ * XCD split, and two work-item sizes.  d_grid and d_out are scratch (their
+ * contents are overwritten).  The winner is cached in the context for the
+ * workload class (grid size, widest kernel, table size, visibility count in
+ * powers of two) and used by tcv_plan_create whenever tuning is NULL. Blocking. */
+tcv_status tcv_autotune(tcv_ctx *ctx, const tcv_geometry *g, const float *d_data,
+                        const float *d_C, float *d_grid, float *d_out, int op,
+                        const tcv_tuning *candidates, int nCandidates, int reps,
+                        tcv_tuning *best, double *bestMs);
 
 /* Tiled gridKernel: each workgroup owns a grid tile in registers, walks the
  * tile's binned visibilities, and writes the tile back once (exclusive tiles)

@@ -297,3 +297,13 @@ def test_profiling_counts_launches(ctx):
     n, ms = ctx.kernel_stats("grid_tiled")
     ctx.set_profiling(False)
     assert n == 1 and ms > 0
+
+
+def test_copy_kernel_bit_exact(ctx):
+    rng = np.random.default_rng(5)
+    for n in (0, 4, 1 << 10, (1 << 20) + 12):
+        a = rng.integers(0, 2**32, n, dtype=np.uint32)
+        src = ctx.to_device(a)
+        dst = ctx.empty(n, np.uint32)
+        dst.copy_from(src)
+        assert np.array_equal(dst.download(), a)

@@ -115,13 +115,17 @@ def test_admission_waits_for_memory_and_cancel(ctx, templates):
         cancelled = True
     except tcv.TcvError:
         cancelled = False      # already admitted: it simply runs
-    res = [srv.wait(r) for r in ids]
-    done = [r for r, i in zip(res, ids) if r["status"] == 0]
+    # results stay in the arena until released, so release each one as it
+    # completes: that is what lets the next request in
+    res = []
+    for r in ids:
+        res.append(srv.wait(r, timeout=60))
+        srv.release(r)
+    done = [r for r in res if r["status"] == 0]
     for a, b in zip(done, done[1:]):
         # one at a time: a request is admitted only after the previous finished
         assert b["admitted"] >= a["completed"] - 1e-4
+    assert len(done) >= 3
     if cancelled:
         assert res[-1]["status"] == 7     # TCV_ERR_CANCELLED
-    for r in ids:
-        srv.release(r)
     srv.close()
