@@ -667,19 +667,28 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled(ItemSource<BATCH> src)
         }
 
         const int col = Bx + lx;
+        const bool colOk = col < gSize;
+        float2 *__restrict__ g0 = jd.grid + (size_t)(By + q0 * RPI + ly) * gSize + col;
+        const int rowsLeft = gSize - (By + q0 * RPI + ly);   // rows j*RPI < rowsLeft exist
+        if (it.w) {
+            // the tile is this item's alone: plain read-modify-write
 #pragma unroll
-        for (int j = 0; j < R; ++j) {
-            const int row = By + (q0 + j) * RPI + ly;
-            if (row < gSize && col < gSize && (ar[j] != 0.f || ai[j] != 0.f)) {
-                float2 *gp = jd.grid + (size_t)row * gSize + col;
-                if (it.w) {
+            for (int j = 0; j < R; ++j) {
+                if (colOk && j * RPI < rowsLeft && (ar[j] != 0.f || ai[j] != 0.f)) {
+                    float2 *gp = g0 + (size_t)j * RPI * gSize;
                     float2 o = *gp;
                     o.x += ar[j];
                     o.y += ai[j];
                     *gp = o;
-                } else {
-                    unsafeAtomicAdd(&gp->x, ar[j]);
-                    unsafeAtomicAdd(&gp->y, ai[j]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                if (colOk && j * RPI < rowsLeft && (ar[j] != 0.f || ai[j] != 0.f)) {
+                    float *gp = reinterpret_cast<float *>(g0 + (size_t)j * RPI * gSize);
+                    unsafeAtomicAdd(gp, ar[j]);
+                    unsafeAtomicAdd(gp + 1, ai[j]);
                 }
             }
         }
@@ -1035,7 +1044,10 @@ tcv_status plan_build_kernels(tcv_ctx *ctx, tcv_plan *p, const GeomDev &g)
 namespace {
 
 constexpr int kNW = 4;   // waves per workgroup
-constexpr int kRB = 4;   // row groups per uniform range test
+#ifndef TCV_RB
+#define TCV_RB 4
+#endif
+constexpr int kRB = TCV_RB;   // row groups per uniform range test (loads issued together)
 
 template <int TW, int TH>
 struct TileCfg {
