@@ -116,6 +116,7 @@ void tcv_destroy(tcv_ctx *ctx)
     if (ctx->timer0) (void)hipEventDestroy(ctx->timer0);
     if (ctx->timer1) (void)hipEventDestroy(ctx->timer1);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+    major_release(ctx);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -611,6 +612,12 @@ tcv_status tcv_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const float *d_data, const 
 {
     if (!ctx || !p || !d_data || !d_C || !d_grid) return tcv_fail(ctx, TCV_ERR_INVALID, "null argument");
     return launch_grid_tiled(ctx, p, d_data, d_C, (float2 *)d_grid, nullptr);
+}
+
+tcv_status tcv_grid_psf(tcv_ctx *ctx, tcv_plan *p, const float *d_C, float *d_grid)
+{
+    if (!ctx || !p || !d_C || !d_grid) return tcv_fail(ctx, TCV_ERR_INVALID, "null argument");
+    return launch_grid_tiled(ctx, p, nullptr, d_C, (float2 *)d_grid, nullptr);
 }
 
 tcv_status tcv_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float *d_grid, const float *d_C,

@@ -25,7 +25,14 @@ struct tcv_ctx {
     unsigned long long *d_counters = nullptr;
     // autotuner results per workload class
     std::map<std::string, tcv_tuning> tuned;
+    // major cycle: FFT plans per grid size, grow-only Hogbom workspace
+    std::map<int, void *> fftPlans;
+    void *majorWs = nullptr;
+    size_t majorWsBytes = 0;
 };
+
+// Frees the major-cycle resources of a context (tcv_major.hip).
+void major_release(tcv_ctx *ctx);
 
 // Workload class of a geometry for the autotuner cache.
 std::string tuning_class(const tcv_geometry *g);

@@ -528,6 +528,7 @@ template <bool HALF>
 __device__ __forceinline__ float2 vis_value(const JobDesc &d, const int4 &en)
 {
     if constexpr (!HALF) {
+        if (!d.data) return make_float2(1.f, 0.f);   // PSF gridding: data == 1 + 0i
         return static_cast<const float2 *>(d.data)[en.z];
     } else {
         const half2_t v = static_cast<const half2_t *>(d.data)[en.z];

@@ -104,6 +104,14 @@ SIGS = {
     "tcv_attic_initC_device": (I, [P, D, D, I, I, I, PI, PD, P]),
     "tcv_mem_info": (I, [P, ct.POINTER(ct.c_size_t), ct.POINTER(ct.c_size_t)]),
     "tcv_copy": (I, [P, P, P, ct.c_size_t]),
+    "tcv_grid_psf": (I, [P, P, P, P]),
+    "tcv_fft_exec": (I, [P, P, I, I]),
+    "tcv_fft_fix": (I, [P, P, LL, ct.c_float]),
+    "tcv_fft_exec_fix": (I, [P, P, I, I, ct.c_float]),
+    "tcv_find_peak": (I, [P, P, LL, ct.POINTER(ct.c_float), ct.POINTER(LL)]),
+    "tcv_hogbom": (I, [P, P, LL, P, LL, P, I, ct.c_float, ct.POINTER(ct.c_float), ct.POINTER(LL),
+                       ct.POINTER(ct.c_float), ct.POINTER(LL)]),
+    "tcv_vis_subtract": (I, [P, P, P, LL]),
     "tcv_autotune": (I, [P, ct.POINTER(Geometry), P, P, P, P, I, ct.POINTER(Tuning), I, I,
                          ct.POINTER(Tuning), PD]),
     "tcv_server_create": (I, [P, ct.POINTER(ServerConfig), ct.POINTER(P)]),

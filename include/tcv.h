@@ -237,6 +237,11 @@ tcv_status tcv_grid_tiled(tcv_ctx *ctx, tcv_plan *plan, const float *d_data,
 tcv_status tcv_degrid_tiled(tcv_ctx *ctx, tcv_plan *plan, const float *d_grid,
                             const float *d_C, float *d_data);
 
+/* gridKernel with isPSF = true (current/tMajorACC/tMajorACC.cc:86-126): every
+ * visibility's value is 1 + 0i, so the grid accumulates the C slices alone
+ * (the point-spread function's uv grid).  Tiled path; accumulates. */
+tcv_status tcv_grid_psf(tcv_ctx *ctx, tcv_plan *plan, const float *d_C, float *d_grid);
+
 /* ---- FP16 storage mode ------------------------------------------------------- */
 /* C(w) taps and visibilities stored as half (re, im) pairs, scaled by the
  * max |component| of their w-plane (taps) or of the whole set (visibilities);
@@ -353,6 +358,41 @@ tcv_status tcv_server_stats(tcv_server *srv, long long *queued, long long *runni
                             size_t *arenaUsed, size_t *arenaCapacity);
 double tcv_server_clock(tcv_server *srv);
 const char *tcv_server_last_error(tcv_server *srv);
+
+
+/* ---- major cycle (current/tMajorACC/tMajorACC.cc) -------------------------
+ * Device-resident neighbours of the gridder in an imaging major cycle.  All
+ * calls are queued on the context stream unless marked blocking. */
+
+/* fftExec (tMajorACC.cc:488-546, GPU form :548-618): in-place unnormalised
+ * 2-D C2C transform of a gSize x gSize complex grid with the origin at pixel
+ * (gSize/2, gSize/2) (the reference rotates by gSize/2 before and after);
+ * forward != 0 uses e^{-i} (FFTW_FORWARD), else e^{+i}.  gSize must be even
+ * (the reference refuses odd sizes too).  hipFFT plans are cached per size. */
+tcv_status tcv_fft_exec(tcv_ctx *ctx, float *d_grid, int gSize, int forward);
+/* fftFix (tMajorACC.cc:620-636): grid = real(grid) * scale, imaginary part 0. */
+tcv_status tcv_fft_fix(tcv_ctx *ctx, float *d_grid, long long nPixels, float scale);
+/* tcv_fft_exec followed by tcv_fft_fix, the fix folded into the transform's
+ * last pass (the reference's inverse-FFT-then-fftFix pair, tMajorACC.cc:1121-1140). */
+tcv_status tcv_fft_exec_fix(tcv_ctx *ctx, float *d_grid, int gSize, int forward, float scale);
+/* findPeak (tMajorACC.cc:689-701): the first pixel of largest |real part| of
+ * a complex image; *maxVal is its (signed) real part.  Blocking. */
+tcv_status tcv_find_peak(tcv_ctx *ctx, const float *d_image, long long nPixels,
+                         float *maxVal, long long *maxPos);
+/* deconvolve (Hogbom CLEAN, tMajorACC.cc:800-840, subtractPsf :723-750):
+ * niters iterations on the complex residual (width x width) with the complex
+ * PSF image (psfWidth x psfWidth), adding gain * peak to the complex model.
+ * Only real parts change, rounded as the reference rounds them; the PSF row
+ * offset follows the reference (ry - px), and PSF rows it would place outside
+ * the PSF are skipped.  No host round trip per iteration.  The four peak
+ * outputs (PSF peak, first residual peak) may be NULL; when any is given the
+ * call blocks until the loop has finished.  Widths below 65536. */
+tcv_status tcv_hogbom(tcv_ctx *ctx, float *d_residual, long long width, const float *d_psf,
+                      long long psfWidth, float *d_model, int niters, float gain,
+                      float *psfPeakVal, long long *psfPeakPos, float *dirtyPeakVal,
+                      long long *dirtyPeakPos);
+/* data = data - model for nVis complex visibilities (tMajorACC.cc:1549-1556). */
+tcv_status tcv_vis_subtract(tcv_ctx *ctx, float *d_data, const float *d_model, long long nVis);
 
 #ifdef __cplusplus
 }

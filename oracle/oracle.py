@@ -64,6 +64,13 @@ def _declare(L):
     L.orc_cpu_baseline.restype = I
     L.orc_cpu_baseline.argtypes = [I, I, I, P, P, P, P, P, P, I, P, I,
                                    ct.POINTER(D), ct.POINTER(D)]
+    LL = ct.c_longlong
+    L.orc_fft_fix.argtypes = [P, LL, ct.c_float]
+    L.orc_find_peak.argtypes = [P, LL, ct.POINTER(ct.c_float), ct.POINTER(LL)]
+    L.orc_subtract_psf.argtypes = [P, LL, P, LL, LL, LL, ct.c_float, ct.c_float]
+    L.orc_deconvolve.argtypes = [P, LL, P, LL, P, I, ct.c_float, ct.POINTER(ct.c_float),
+                                 ct.POINTER(LL), ct.POINTER(ct.c_float), ct.POINTER(LL)]
+    L.orc_vis_subtract.argtypes = [P, P, LL]
 
 
 def _p(a):
@@ -291,3 +298,77 @@ def cpu_baseline(nThreads, nVis, data, iu, iv, cOffset, C, gSize, sSize, wPlane=
     if rc:
         raise MemoryError(f"cpu baseline failed (rc={rc})")
     return tg.value, td.value
+
+
+# ---- major cycle (current/tMajorACC/tMajorACC.cc) -------------------------------
+
+def fft_shift_copy(grid):
+    """The reference's rotate-by-copy around its FFTs: out[(c + g/2) % g][r] =
+    in[c][(r + g/2) % g] for every row c (tMajorACC.cc:509-520), written as
+    explicit index loops over rows; even g only."""
+    g = grid.shape[0]
+    if g % 2:
+        raise ValueError("fftExec does not support odd sizes")
+    out = np.empty_like(grid)
+    h = g // 2
+    for col in range(g):
+        colout = (col + h) % g
+        out[colout, :h] = grid[col, h:]
+        out[colout, h:] = grid[col, :h]
+    return out
+
+
+def fft_exec(grid, forward):
+    """fftExec: shift, unnormalised 2-D C2C transform (FFTW sign convention:
+    forward e^-i, backward e^+i), shift back -- tMajorACC.cc:488-546.  The
+    transform runs in complex128 (numpy's pocketfft); returns complex128."""
+    g = np.asarray(grid).astype(np.complex128)
+    b = fft_shift_copy(g)
+    b = np.fft.fft2(b) if forward else np.fft.ifft2(b) * (b.shape[0] * b.shape[1])
+    return fft_shift_copy(b)
+
+
+def fft_fix(grid, scale):
+    """fftFix (tMajorACC.cc:620-627) on a complex64 array, in a copy."""
+    g = _c(grid, np.complex64).copy()
+    lib().orc_fft_fix(_p(g), g.size, ct.c_float(scale))
+    return g
+
+
+def find_peak(image):
+    """findPeak (tMajorACC.cc:689-701): (signed real value, flat position)."""
+    im = _c(image, np.complex64)
+    v, pos = ct.c_float(), ct.c_longlong()
+    lib().orc_find_peak(_p(im), im.size, ct.byref(v), ct.byref(pos))
+    return v.value, pos.value
+
+
+def subtract_psf(psf, residual, peakPos, psfPeakPos, absPeakVal, gain=0.1):
+    """subtractPsf (tMajorACC.cc:723-750) on a copy of residual."""
+    psf = _c(psf, np.complex64)
+    r = _c(residual, np.complex64).copy()
+    lib().orc_subtract_psf(_p(psf), psf.shape[0], _p(r), r.shape[0], int(peakPos),
+                           int(psfPeakPos), ct.c_float(absPeakVal), ct.c_float(gain))
+    return r
+
+
+def deconvolve(residual, psf, model=None, niters=100, gain=0.1):
+    """deconvolve (Hogbom, tMajorACC.cc:800-840) on copies; returns (residual,
+    model, (psfPeakVal, psfPeakPos), (dirtyPeakVal, dirtyPeakPos))."""
+    psf = _c(psf, np.complex64)
+    r = _c(residual, np.complex64).copy()
+    m = np.zeros_like(r) if model is None else _c(model, np.complex64).copy()
+    pv, dv = ct.c_float(), ct.c_float()
+    pp, dp = ct.c_longlong(), ct.c_longlong()
+    lib().orc_deconvolve(_p(r), r.shape[0], _p(psf), psf.shape[0], _p(m), int(niters),
+                         ct.c_float(gain), ct.byref(pv), ct.byref(pp), ct.byref(dv),
+                         ct.byref(dp))
+    return r, m, (pv.value, pp.value), (dv.value, dp.value)
+
+
+def vis_subtract(data, model):
+    """data - model (tMajorACC.cc:1549-1551), in a copy."""
+    d = _c(data, np.complex64).copy()
+    m = _c(model, np.complex64)
+    lib().orc_vis_subtract(_p(d), _p(m), d.size)
+    return d

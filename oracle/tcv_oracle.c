@@ -561,3 +561,86 @@ int orc_cpu_baseline(int nThreads, int nVis, int nInput, const float *data,
     *tDegrid = t3 - t2;
     return ok ? 0 : 2;
 }
+
+/* ---- major cycle --------------------------------------------------------- */
+
+/* current/tMajorACC/tMajorACC.cc:620-627 */
+void orc_fft_fix(float *grid, long long nPixels, float scale)
+{
+    for (long long i = 0; i < nPixels; i++) {
+        grid[2 * i] = grid[2 * i] * scale;
+        grid[2 * i + 1] = 0.0f;
+    }
+}
+
+/* current/tMajorACC/tMajorACC.cc:689-701 */
+void orc_find_peak(const float *image, long long nPixels, float *maxVal,
+                   long long *maxPos)
+{
+    float mv = 0.0f;
+    long long mp = 0;
+    for (long long i = 0; i < nPixels; i++) {
+        const float v = image[2 * i];
+        if (fabsf(v) > fabsf(mv)) {
+            mv = v;
+            mp = i;
+        }
+    }
+    *maxVal = mv;
+    *maxPos = mp;
+}
+
+/* current/tMajorACC/tMajorACC.cc:723-750 (idxToPos / posToIdx :677-687) */
+void orc_subtract_psf(const float *psf, long long psfWidth, float *residual,
+                      long long residualWidth, long long peakPos,
+                      long long psfPeakPos, float absPeakVal, float gain)
+{
+    const long long rx = peakPos % residualWidth, ry = peakPos / residualWidth;
+    const long long px = psfPeakPos % psfWidth, py = psfPeakPos / psfWidth;
+    const long long diffx = rx - px;
+    const long long diffy = ry - px;   /* as in the reference (not ry - py) */
+    const long long startx = rx - px > 0 ? rx - px : 0;
+    const long long starty = ry - py > 0 ? ry - py : 0;
+    long long stopx = rx + (psfWidth - px - 1), stopy = ry + (psfWidth - py - 1);
+    if (stopx > residualWidth - 1) stopx = residualWidth - 1;
+    if (stopy > residualWidth - 1) stopy = residualWidth - 1;
+    const float amp = gain * absPeakVal;
+    for (long long y = starty; y <= stopy; ++y) {
+        const long long sy = y - diffy;
+        if (sy < 0 || sy >= psfWidth) continue;
+        for (long long x = startx; x <= stopx; ++x) {
+            float *r = residual + 2 * (y * residualWidth + x);
+            r[0] = r[0] - amp * psf[2 * (sy * psfWidth + (x - diffx))];
+        }
+    }
+}
+
+/* current/tMajorACC/tMajorACC.cc:800-840 */
+void orc_deconvolve(float *residual, long long dirtyWidth, const float *psf,
+                    long long psfWidth, float *model, int niters, float gain,
+                    float *psfPeakVal, long long *psfPeakPos,
+                    float *dirtyPeakVal, long long *dirtyPeakPos)
+{
+    float pv;
+    long long pp;
+    orc_find_peak(psf, psfWidth * psfWidth, &pv, &pp);
+    if (psfPeakVal) *psfPeakVal = pv;
+    if (psfPeakPos) *psfPeakPos = pp;
+    for (int i = 0; i < niters; ++i) {
+        float av;
+        long long ap;
+        orc_find_peak(residual, dirtyWidth * dirtyWidth, &av, &ap);
+        if (i == 0) {
+            if (dirtyPeakVal) *dirtyPeakVal = av;
+            if (dirtyPeakPos) *dirtyPeakPos = ap;
+        }
+        model[2 * ap] = model[2 * ap] + av * gain;
+        orc_subtract_psf(psf, psfWidth, residual, dirtyWidth, ap, pp, av, gain);
+    }
+}
+
+/* current/tMajorACC/tMajorACC.cc:1549-1551 */
+void orc_vis_subtract(float *data, const float *model, long long nVis)
+{
+    for (long long i = 0; i < 2 * nVis; i++) data[i] = data[i] - model[i];
+}

@@ -23,6 +23,15 @@
  *   orc_grid / orc_degrid single-visibility patch == C slice, linearity,
  *                         bilinear adjointness (FP64), grid==1 => sum of slice,
  *                         brute force in FP64, compiled-reference checksums
+ *   orc_find_peak         numpy argmax of |re| (first occurrence), ties, zeros
+ *   orc_subtract_psf      delta PSF changes one pixel by gain*peak; all-ones
+ *                         PSF changes exactly the overlap rectangle; the
+ *                         reference's row offset with an off-diagonal peak
+ *   orc_deconvolve        closed form on a delta PSF (peak * 0.9^n), model sum
+ *                         + residual conservation on a delta PSF
+ *   orc_fft_fix, orc_vis_subtract   element-wise closed forms
+ * (tMajorACC needs FFTW and an OpenACC compiler, neither present here, so no
+ * compiled reference backs the major-cycle functions.)
  */
 #ifndef TCV_ORACLE_H
 #define TCV_ORACLE_H
@@ -121,6 +130,40 @@ int orc_cpu_baseline(int nThreads, int nVis, int nInput, const float *data,
                      const int *iu, const int *iv, const int *cOffset,
                      const int *wPlane, const int *sSize, int sSizeUniform,
                      const float *C, int gSize, double *tGrid, double *tDegrid);
+
+/* ---- major cycle (current/tMajorACC/tMajorACC.cc) ---------------------- */
+
+/* fftFix: grid[i] = real(grid[i]) * scale (imaginary part set to 0) --
+ * current/tMajorACC/tMajorACC.cc:620-627 */
+void orc_fft_fix(float *grid, long long nPixels, float scale);
+
+/* findPeak: the first pixel of largest |real part|; *maxVal is that real part
+ * (signed), 0 and position 0 for an all-zero image --
+ * current/tMajorACC/tMajorACC.cc:689-701 */
+void orc_find_peak(const float *image, long long nPixels, float *maxVal,
+                   long long *maxPos);
+
+/* subtractPsf: residual -= gain * peak * psf (real parts), psf placed with
+ * its peak on the residual peak; the row offset uses the reference's
+ * diffy = ry - px (tMajorACC.cc:735); PSF rows that this offset puts outside
+ * the PSF are skipped (the reference would read outside its buffer) --
+ * current/tMajorACC/tMajorACC.cc:723-750 */
+void orc_subtract_psf(const float *psf, long long psfWidth, float *residual,
+                      long long residualWidth, long long peakPos,
+                      long long psfPeakPos, float absPeakVal, float gain);
+
+/* deconvolve: Hogbom CLEAN, niters iterations of findPeak / model update /
+ * subtractPsf with the given gain (0.1 in the reference) --
+ * current/tMajorACC/tMajorACC.cc:800-840.  Optional outputs: the PSF peak and
+ * the first residual peak (the values the reference prints). */
+void orc_deconvolve(float *residual, long long dirtyWidth, const float *psf,
+                    long long psfWidth, float *model, int niters, float gain,
+                    float *psfPeakVal, long long *psfPeakPos,
+                    float *dirtyPeakVal, long long *dirtyPeakPos);
+
+/* visibility update: data[i] = data[i] - model[i] --
+ * current/tMajorACC/tMajorACC.cc:1549-1551 */
+void orc_vis_subtract(float *data, const float *model, long long nVis);
 
 #ifdef __cplusplus
 }
