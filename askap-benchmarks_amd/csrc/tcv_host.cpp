@@ -423,6 +423,7 @@ static void auto_tuning(int maxS, const tcv_geometry *g, tcv_tuning *t)
     if (t->slice_keys <= 0) t->slice_keys = (int)std::min<long long>(8192, slices);
     if (t->phases <= 0) t->phases = default_phases(g->cSize, t->slice_keys);
     if (t->xcd_split < 0) t->xcd_split = (t->phases >= 8 && t->phases % 8 == 0) ? 1 : 0;
+    if (t->kernel <= 0) t->kernel = kDefaultTileKernel;
 }
 
 static void free_plan(tcv_plan *p)
@@ -444,7 +445,7 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
     if (g->gSize > 65535) return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tiled path needs gSize <= 65535");
     if (g->cSize * 8 >= 0x7fffffffLL)
         return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tiled path needs a C table below 2 GiB");
-    tcv_tuning t = tuning ? *tuning : tcv_tuning{0, 0, 0, 0, 0, -1};
+    tcv_tuning t = tuning ? *tuning : tcv_tuning{0, 0, 0, 0, 0, -1, 0};
     if (!tuning && ctx) {
         auto it = ctx->tuned.find(tuning_class(g));
         if (it != ctx->tuned.end()) t = it->second;
@@ -461,6 +462,9 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
     sh.k2 = std::max(1, t.slice_keys);
     sh.phases = std::min(t.phases, sh.k2);
     sh.xcd = (t.xcd_split > 0 && sh.phases % 8 == 0) ? 1 : 0;
+    if (t.kernel != 1 && t.kernel != 2)
+        return tcv_fail(ctx, TCV_ERR_INVALID, "tuning.kernel must be 0, 1 or 2");
+    sh.kern = t.kernel;
     sh.nTX = (g->gSize + sh.tw - 1) / sh.tw;
     sh.nTY = (g->gSize + sh.th - 1) / sh.th;
     sh.nTiles = sh.nTX * sh.nTY;
@@ -597,7 +601,7 @@ tcv_status tcv_plan_info(tcv_ctx *ctx, tcv_plan *p, tcv_tuning *t, long long *en
                          long long *items, long long *outOfRange)
 {
     if (!ctx || !p) return TCV_ERR_INVALID;
-    if (t) *t = tcv_tuning{p->sh.tw, p->sh.th, p->sh.chunk, p->sh.k2, p->sh.phases, p->sh.xcd};
+    if (t) *t = tcv_tuning{p->sh.tw, p->sh.th, p->sh.chunk, p->sh.k2, p->sh.phases, p->sh.xcd, p->sh.kern};
     int meta[4] = {0, 0, 0, 0};   // items, queue head, entries, out of range
     TCV_HIP(ctx, hipMemcpyAsync(meta, p->meta, sizeof(meta), hipMemcpyDeviceToHost, ctx->stream));
     TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -720,9 +724,11 @@ tcv_status tcv_batch_create(tcv_ctx *ctx, int nJobs, const tcv_job *jobs, tcv_ba
         if (j == 0) {
             b->tw = J.plan->sh.tw;
             b->th = J.plan->sh.th;
-        } else if (J.plan->sh.tw != b->tw || J.plan->sh.th != b->th) {
+            b->kern = J.plan->sh.kern;
+        } else if (J.plan->sh.tw != b->tw || J.plan->sh.th != b->th || J.plan->sh.kern != b->kern) {
             delete b;
-            return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_batch_create: all plans of a batch need one tile shape");
+            return tcv_fail(ctx, TCV_ERR_INVALID,
+                            "tcv_batch_create: all plans of a batch need one tile shape and kernel");
         }
         b->plans.push_back(J.plan);
         b->host.push_back(make_job_desc(J.plan, J.d_data, J.d_C, (float2 *)J.d_grid, (float2 *)J.d_out, nullptr));
@@ -875,10 +881,11 @@ static std::vector<tcv_tuning> builtin_candidates(const tcv_geometry *g)
     for (auto &t : tiles)
         for (int p : {ph / 2, ph, ph * 2}) {
             if (p < 1 || p > k2) continue;
-            c.push_back(tcv_tuning{t.first, t.second, 2048, k2, p, p >= 8 ? 1 : 0});
+            for (int kern : {1, 2})
+                c.push_back(tcv_tuning{t.first, t.second, 2048, k2, p, p >= 8 ? 1 : 0, kern});
         }
-    c.push_back(tcv_tuning{tiles[0].first, tiles[0].second, 512, k2, ph, ph >= 8 ? 1 : 0});
-    c.push_back(tcv_tuning{tiles[0].first, tiles[0].second, 2048, 1, 1, 0});   // no ordering
+    c.push_back(tcv_tuning{tiles[0].first, tiles[0].second, 512, k2, ph, ph >= 8 ? 1 : 0, 0});
+    c.push_back(tcv_tuning{tiles[0].first, tiles[0].second, 2048, 1, 1, 0, 0});   // no ordering
     return c;
 }
 

@@ -103,8 +103,12 @@ struct JobDesc {
     int nVis;
 };
 
+// Tiled kernel used when a tuning leaves the choice open (1 registers, 2 LDS).
+constexpr int kDefaultTileKernel = 1;
+
 struct PlanShape {
     int tw, th, chunk, k2, phases, xcd;
+    int kern;              // 1 register tile, 2 LDS tile
     int nTX, nTY, nTiles;
     int maxS;
     long long maxEntries;  // upper bound on visibility x tile pairs
@@ -143,7 +147,7 @@ void plan_bind(tcv_plan *p, char *block);
 
 struct tcv_batch {
     int nJobs = 0;
-    int tw = 0, th = 0;
+    int tw = 0, th = 0, kern = kDefaultTileKernel;
     bool half = false;
     std::vector<JobDesc> host;
     std::vector<tcv_plan *> plans;

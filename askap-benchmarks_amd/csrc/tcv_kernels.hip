@@ -790,6 +790,198 @@ __global__ __launch_bounds__(NW * 64) void k_degrid_tiled(ItemSource<BATCH> src)
     }
 }
 
+// ---- tiled gridder / degridder, LDS tile with dense lanes --------------------
+// The same work items as above, with the tile in LDS so that lanes map to the
+// points of each visibility's footprint on the tile rather than to tile
+// columns: an entry covering w x h points of the tile occupies ceil(w*h/64)
+// lane groups instead of h full 64-lane rows (no lanes idle at the footprint's
+// left and right edges).  Gridder: wave k owns the tile rows r with r % 4 == k
+// (zero, accumulate and flush them), so no barrier or LDS atomic is needed and
+// every cell accumulates the item's entries in entry order, rounded exactly as
+// the register-tile gridder rounds them.
+#ifndef TCV_LDS_U
+#define TCV_LDS_U 4
+#endif
+constexpr int kLdsU = TCV_LDS_U;   // footprint points per lane whose taps load together
+
+__device__ __forceinline__ void split_flat(int p, float rw, int w, int &row, int &col)
+{
+    // p < 2^12, w <= 64: (p + 0.5) / w is at least 1/(2w) away from an integer,
+    // far beyond the rounding of the float reciprocal
+    row = (int)(((float)p + 0.5f) * rw);
+    col = p - row * w;
+}
+
+template <int TW, int TH, bool HALF, bool BATCH>
+__global__ __launch_bounds__(256) void k_grid_lds(ItemSource<BATCH> src)
+{
+    constexpr unsigned TB = Taps<HALF>::kBytes;
+    constexpr int OWN = (TH / 4) * TW;   // cells owned by one wave
+    __shared__ float2 tile[TH * TW];
+    __shared__ int s_item;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nItems = src.total();
+
+    for (;;) {
+        if (threadIdx.x == 0) s_item = src.acquire();
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (item >= nItems) break;
+        JobDesc jd;
+        int local;
+        src.locate(item, jd, local);
+        const __amdgpu_buffer_rsrc_t rc = make_rsrc(jd.C, jd.cBytes);
+        const int4 *__restrict__ entries = jd.entries;
+        const int gSize = jd.gSize;
+        const int4 it = jd.items[local];
+        const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
+        const int Bx = txi * TW, By = tyi * TH;
+
+        // own rows r = wave + 4 * (q / TW), column q % TW
+        for (int q = lane; q < OWN; q += 64)
+            tile[(wave + 4 * (q / TW)) * TW + (q % TW)] = make_float2(0.f, 0.f);
+
+        for (int e = it.y; e < it.y + it.z; ++e) {
+            const int4 en = entries[e];
+            const float2 d = vis_value<HALF>(jd, en);
+            const int u0 = en.x & 0xFFFF, v0 = (int)((unsigned)en.x >> 16);
+            const int c0 = en.y, s = entry_width(en.w);
+            const int colLo = max(u0 - Bx, 0), colHi = min(u0 + s - Bx, TW);
+            const int rowLo = max(v0 - By, 0), rowHi = min(v0 + s - By, TH);
+            const int w = colHi - colLo;
+            const int r0 = rowLo + ((wave - rowLo) & 3);   // first own row in range
+            if (w <= 0 || r0 >= rowHi) continue;
+            const int n = w * ((rowHi - r0 + 3) >> 2);
+            const float rw = 1.0f / (float)w;
+            const int cBase = c0 + (By + r0 - v0) * s + (Bx + colLo - u0);
+            float2 *__restrict__ tBase = tile + r0 * TW + colLo;
+            // kLdsU points per lane per pass: their taps are loaded together
+            for (int p0 = lane; p0 < n; p0 += 64 * kLdsU) {
+                float2 c[kLdsU];
+                int to[kLdsU];
+#pragma unroll
+                for (int u = 0; u < kLdsU; ++u) {
+                    const int p = p0 + 64 * u;
+                    int rl, col;
+                    split_flat(p, rw, w, rl, col);
+                    to[u] = p < n ? 4 * rl * TW + col : -1;
+                    c[u] = Taps<HALF>::load(rc, p < n ? (unsigned)(cBase + 4 * rl * s + col) * TB : kOOB);
+                }
+#pragma unroll
+                for (int u = 0; u < kLdsU; ++u) {
+                    if (to[u] >= 0) {
+                        float2 *t = tBase + to[u];
+                        float2 v = *t;
+                        v.x = fmaf(d.x, c[u].x, fmaf(-d.y, c[u].y, v.x));
+                        v.y = fmaf(d.x, c[u].y, fmaf(d.y, c[u].x, v.y));
+                        *t = v;
+                    }
+                }
+            }
+        }
+
+        for (int q = lane; q < OWN; q += 64) {
+            const int r = wave + 4 * (q / TW), cc = q % TW;
+            const int row = By + r, col = Bx + cc;
+            const float2 v = tile[r * TW + cc];
+            if (row < gSize && col < gSize && (v.x != 0.f || v.y != 0.f)) {
+                float2 *gp = jd.grid + (size_t)row * gSize + col;
+                if (it.w) {
+                    float2 o = *gp;
+                    o.x += v.x;
+                    o.y += v.y;
+                    *gp = o;
+                } else {
+                    unsafeAtomicAdd(&gp->x, v.x);
+                    unsafeAtomicAdd(&gp->y, v.y);
+                }
+            }
+        }
+    }
+}
+
+// Degridder with the tile in LDS: the workgroup loads the tile once per item,
+// waves take whole entries, lanes walk the entry's footprint on the tile as a
+// flat index, and the per-visibility sum is reduced with DPP as above.
+template <int TW, int TH, bool HALF, bool BATCH>
+__global__ __launch_bounds__(256) void k_degrid_lds(ItemSource<BATCH> src)
+{
+    constexpr unsigned TB = Taps<HALF>::kBytes;
+    __shared__ float2 tile[TH * TW];
+    __shared__ int s_item;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nItems = src.total();
+
+    for (;;) {
+        if (threadIdx.x == 0) s_item = src.acquire();
+        __syncthreads();   // also: every wave is done with the previous tile
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= nItems) break;
+        JobDesc jd;
+        int local;
+        src.locate(item, jd, local);
+        const __amdgpu_buffer_rsrc_t rc = make_rsrc(jd.C, jd.cBytes);
+        const int4 *__restrict__ entries = jd.entries;
+        const float2 *__restrict__ grid = jd.grid;
+        const int gSize = jd.gSize;
+        const int4 it = jd.items[local];
+        const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
+        const int Bx = txi * TW, By = tyi * TH;
+        for (int q = threadIdx.x; q < TH * TW; q += 256) {
+            const int row = By + q / TW, col = Bx + q % TW;
+            tile[q] = (row < gSize && col < gSize) ? grid[(size_t)row * gSize + col]
+                                                   : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+
+        for (int e = it.y + wave; e < it.y + it.z; e += 4) {
+            const int4 en = entries[e];
+            const int u0 = en.x & 0xFFFF, v0 = (int)((unsigned)en.x >> 16);
+            const int c0 = en.y, s = entry_width(en.w);
+            const int colLo = max(u0 - Bx, 0), colHi = min(u0 + s - Bx, TW);
+            const int rowLo = max(v0 - By, 0), rowHi = min(v0 + s - By, TH);
+            const int w = colHi - colLo, h = rowHi - rowLo;
+            if (w <= 0 || h <= 0) continue;
+            const int n = w * h;
+            const float rw = 1.0f / (float)w;
+            const int cBase = c0 + (By + rowLo - v0) * s + (Bx + colLo - u0);
+            const float2 *__restrict__ tBase = tile + rowLo * TW + colLo;
+            float sr = 0.f, si = 0.f;
+            for (int p0 = lane; p0 < n; p0 += 64 * kLdsU) {
+                float2 c[kLdsU], g[kLdsU];
+#pragma unroll
+                for (int u = 0; u < kLdsU; ++u) {
+                    const int p = p0 + 64 * u;
+                    int rl, col;
+                    split_flat(p, rw, w, rl, col);
+                    c[u] = Taps<HALF>::load(rc, p < n ? (unsigned)(cBase + rl * s + col) * TB : kOOB);
+                    g[u] = p < n ? tBase[rl * TW + col] : make_float2(0.f, 0.f);
+                }
+#pragma unroll
+                for (int u = 0; u < kLdsU; ++u) {
+                    sr = fmaf(g[u].x, c[u].x, fmaf(-g[u].y, c[u].y, sr));
+                    si = fmaf(g[u].x, c[u].y, fmaf(g[u].y, c[u].x, si));
+                }
+            }
+            sr = wave_sum(sr);
+            si = wave_sum(si);
+            if constexpr (HALF) {
+                const float ps = jd.planeScale[entry_plane(en, jd)];
+                sr *= ps;
+                si *= ps;
+            }
+            if (lane == 0) {
+                float *op = reinterpret_cast<float *>(jd.out + en.z);
+                unsafeAtomicAdd(op, sr);
+                unsafeAtomicAdd(op + 1, si);
+            }
+        }
+    }
+}
+
 // ---- FP16 storage conversion ----------------------------------------------------
 // Per-plane max |component| of C, then taps / scale rounded to half.  Each thread
 // walks a contiguous run of taps and flushes its running max with one atomicMax
@@ -1080,10 +1272,11 @@ __global__ void k_batch_prefix(const JobDesc *__restrict__ jobs, int nJobs, int 
 }
 
 template <int TW, int TH, bool HALF, bool BATCH>
-tcv_status grid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src)
+tcv_status grid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_kind)
 {
     using T = TileCfg<TW, TH>;
-    auto kern = k_grid_tiled<TW, T::RG, kNW, (T::RG % kRB == 0 ? kRB : T::RG), HALF, BATCH>;
+    auto kern = kern_kind == 2 ? k_grid_lds<TW, TH, HALF, BATCH>
+                               : k_grid_tiled<TW, T::RG, kNW, (T::RG % kRB == 0 ? kRB : T::RG), HALF, BATCH>;
     const int blocks = persistent_blocks(ctx, (const void *)kern, kNW * 64);
     ProfScope ps(ctx, BATCH ? "grid_batch" : (HALF ? "grid_tiled_half" : "grid_tiled"));
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kNW * 64), 0, ctx->stream, src);
@@ -1092,10 +1285,11 @@ tcv_status grid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src)
 }
 
 template <int TW, int TH, bool HALF, bool BATCH>
-tcv_status degrid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src)
+tcv_status degrid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_kind)
 {
     using T = TileCfg<TW, TH>;
-    auto kern = k_degrid_tiled<TW, T::RD, kNW, (T::RD % kRB == 0 ? kRB : T::RD), HALF, BATCH>;
+    auto kern = kern_kind == 2 ? k_degrid_lds<TW, TH, HALF, BATCH>
+                               : k_degrid_tiled<TW, T::RD, kNW, (T::RD % kRB == 0 ? kRB : T::RD), HALF, BATCH>;
     const int blocks = persistent_blocks(ctx, (const void *)kern, kNW * 64);
     ProfScope ps(ctx, BATCH ? "degrid_batch" : (HALF ? "degrid_tiled_half" : "degrid_tiled"));
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kNW * 64), 0, ctx->stream, src);
@@ -1146,8 +1340,8 @@ tcv_status launch_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const void *data, const 
     TCV_HIP(ctx, hipGetLastError());
 #define X(a, b)                                                                        \
     if (p->sh.tw == a && p->sh.th == b)                                                \
-        return half ? grid_tiled_t<a, b, true, false>(ctx, src)                        \
-                    : grid_tiled_t<a, b, false, false>(ctx, src);
+        return half ? grid_tiled_t<a, b, true, false>(ctx, src, p->sh.kern)            \
+                    : grid_tiled_t<a, b, false, false>(ctx, src, p->sh.kern);
     TCV_TILE_CONFIGS(X)
 #undef X
     return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");
@@ -1164,8 +1358,8 @@ tcv_status launch_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *grid, co
     TCV_HIP(ctx, hipGetLastError());
 #define X(a, b)                                                                        \
     if (p->sh.tw == a && p->sh.th == b)                                                \
-        return half ? degrid_tiled_t<a, b, true, false>(ctx, src)                      \
-                    : degrid_tiled_t<a, b, false, false>(ctx, src);
+        return half ? degrid_tiled_t<a, b, true, false>(ctx, src, p->sh.kern)          \
+                    : degrid_tiled_t<a, b, false, false>(ctx, src, p->sh.kern);
     TCV_TILE_CONFIGS(X)
 #undef X
     return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");
@@ -1185,8 +1379,8 @@ tcv_status launch_batch(tcv_ctx *ctx, tcv_batch *b, bool degrid)
     // batches run FP32 storage
 #define X(a, c)                                                                        \
     if (b->tw == a && b->th == c)                                                      \
-        return degrid ? degrid_tiled_t<a, c, false, true>(ctx, src)                    \
-                      : grid_tiled_t<a, c, false, true>(ctx, src);
+        return degrid ? degrid_tiled_t<a, c, false, true>(ctx, src, b->kern)           \
+                      : grid_tiled_t<a, c, false, true>(ctx, src, b->kern);
     TCV_TILE_CONFIGS(X)
 #undef X
     return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");

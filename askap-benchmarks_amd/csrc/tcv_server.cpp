@@ -261,7 +261,8 @@ tcv_status tcv_server_create(tcv_ctx *ctx, const tcv_server_config *cfg, tcv_ser
     auto s = std::make_unique<tcv_server>();
     s->ctx = ctx;
     s->cfg = *cfg;
-    s->tuning = tcv_tuning{cfg->tile_w, cfg->tile_h, cfg->chunk, cfg->slice_keys, cfg->phases, 0};
+    s->tuning = tcv_tuning{cfg->tile_w, cfg->tile_h, cfg->chunk, cfg->slice_keys, cfg->phases, 0,
+                           kDefaultTileKernel};
     if (s->tuning.tile_w <= 0) s->tuning.tile_w = 64;   // one tile shape for every batch
     if (s->tuning.tile_h <= 0) s->tuning.tile_h = 64;
     TCV_HIP(ctx, hipSetDevice(ctx->device));
@@ -279,6 +280,7 @@ tcv_status tcv_server_create(tcv_ctx *ctx, const tcv_server_config *cfg, tcv_ser
         for (tcv_batch *b : {&sl.grid, &sl.degrid}) {
             b->tw = s->tuning.tile_w;
             b->th = s->tuning.tile_h;
+            b->kern = s->tuning.kernel;
             if (hipMalloc((void **)&b->d_jobs, sizeof(JobDesc) * maxJobs) != hipSuccess ||
                 hipMalloc((void **)&b->d_prefix, sizeof(int) * (maxJobs + 1)) != hipSuccess ||
                 hipMalloc((void **)&b->d_queue, 256) != hipSuccess)

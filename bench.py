@@ -43,7 +43,7 @@ def alg_bytes(points, nVis, gSize, op):
 
 
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_grid_degrid_tiled_spectral.md")
-TUNING_KEYS = ("tile_w", "tile_h", "chunk", "slice_keys", "phases", "xcd_split")
+TUNING_KEYS = ("tile_w", "tile_h", "chunk", "slice_keys", "phases", "xcd_split", "kernel")
 
 
 def measured_traffic(kernel, info):
@@ -141,6 +141,36 @@ def bench_batch(ctx, args):
     return out
 
 
+def bench_major(ctx, args):
+    """tMajorACC's imaging loop at its own scale (160 000 samples, 4096^2
+    grid, wSize 33, 5 major cycles of 100 Hogbom iterations), device-resident:
+    per-stage times summed over the cycles (each stage synchronised)."""
+    from tcv.major import MajorCycle
+    t0 = time.time()
+    mc = MajorCycle(ctx, nSamples=args.major_samples, nMajor=args.major_cycles,
+                    nMinor=args.major_minor)
+    setup_s = time.time() - t0
+    mc.cycle(0)                       # warm: plans, FFT plans, workspaces
+    ctx.sync()
+    mc = MajorCycle(ctx, nSamples=args.major_samples, nMajor=args.major_cycles,
+                    nMinor=args.major_minor)
+    ctx.sync()
+    w0 = time.perf_counter()
+    mc.run(timed=True)
+    wall = time.perf_counter() - w0
+    (pv, pp), (dv, dp) = mc.peaks[0]
+    out = {"workload": f"tMajorACC: {args.major_samples} samples x 1 channel, 4096^2 grid, "
+                       f"wSize 33, {args.major_cycles} major x {args.major_minor} minor cycles",
+           "total_ms": wall * 1e3, "per_cycle_ms": wall * 1e3 / args.major_cycles,
+           "stages_ms": {k: v * 1e3 for k, v in mc.times.items()},
+           "setup_s": setup_s,
+           "psf_peak": [pv, pp], "first_dirty_peak": [dv, dp]}
+    log(f"major: {out['per_cycle_ms']:.1f} ms per cycle; stages "
+        + ", ".join(f"{k} {v:.1f}" for k, v in out["stages_ms"].items()))
+    del mc
+    return out
+
+
 def bench_serving(ctx, args, capacity):
     """Poisson request streams against the continuous-batching server.
 
@@ -230,6 +260,7 @@ def main():
     ap.add_argument("--slice-keys", type=int, default=0)
     ap.add_argument("--phases", type=int, default=0)
     ap.add_argument("--xcd-split", type=int, default=-1)
+    ap.add_argument("--kernel", type=int, default=0, help="1 register tile, 2 LDS tile, 0 auto")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -237,6 +268,10 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--no-major", action="store_true")
+    ap.add_argument("--major-samples", type=int, default=160000)
+    ap.add_argument("--major-cycles", type=int, default=5)
+    ap.add_argument("--major-minor", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: the gridding path here runs on a single "
@@ -252,7 +287,8 @@ def main():
         f"C {prob.C.nbytes / 2**30:.2f} GiB")
     ctx = tcv.Context(0)
     tuning = dict(tile_w=args.tile_w, tile_h=args.tile_h, chunk=args.chunk,
-                  slice_keys=args.slice_keys, phases=args.phases, xcd_split=args.xcd_split)
+                  slice_keys=args.slice_keys, phases=args.phases, xcd_split=args.xcd_split,
+                  kernel=args.kernel)
     job = tcv.GridJob(ctx, prob, tuning=tuning)
     assert job.nBad == 0, f"{job.nBad} visibilities out of range"
     points = job.points
@@ -333,6 +369,9 @@ def main():
         result["batch"] = bench_batch(ctx, args)
         if not args.no_serve:
             result["serving"] = bench_serving(ctx, args, result["batch"]["batched"]["value"])
+
+    if not args.no_major:
+        result["major_cycle"] = bench_major(ctx, args)
 
     if not args.no_cpu:
         import oracle as O

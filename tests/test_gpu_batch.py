@@ -95,3 +95,15 @@ def test_device_initC_matches_host_taps(ctx):
     assert np.abs(ratio[k] / r - 1).max() < 1e-5
     # the device table is normalised to sum |C| = wSize * os^2 in FP64
     assert np.abs(d.astype(np.complex128)).sum() == pytest.approx(33 * 64, rel=1e-6)
+
+
+def test_batch_lds_kernel_vs_oracle(ctx):
+    bj = BatchJobs(ctx, small_specs(4, seed=21), tuning=dict(kernel=2))
+    bj.zero_grids()
+    bj.run_batched(graph=True)
+    for j in range(len(bj.jobs)):
+        g = bj.jobs[j]["grid"].download()
+        gref, _ = oracle_job(bj, j)
+        assert rel_err(g, gref) <= TOL, j
+        _, oref = oracle_job(bj, j, grid_in=g)
+        assert rel_err(bj.jobs[j]["out"].download(), oref) <= TOL, j

@@ -17,11 +17,11 @@ pytestmark = pytest.mark.gpu
 HALF_TOL = 2e-3
 
 
-def _half_run(ctx, vis, data, C, nPlanes, planeTaps, gSize, planeStart=None):
+def _half_run(ctx, vis, data, C, nPlanes, planeTaps, gSize, planeStart=None, kernel=0):
     d_C, d_data = ctx.to_device(C), ctx.to_device(data)
     Ch = tcv.HalfTable(ctx, d_C, nPlanes, planeTaps, planeStart)
     dh = tcv.HalfVis(ctx, d_data)
-    plan = tcv.Plan(ctx, vis)
+    plan = tcv.Plan(ctx, vis, kernel=kernel)
     g = ctx.zeros((gSize, gSize), np.complex64)
     plan.grid_half(dh, Ch, g)
     out = ctx.empty(vis.nVis, np.complex64)
@@ -29,11 +29,12 @@ def _half_run(ctx, vis, data, C, nPlanes, planeTaps, gSize, planeStart=None):
     return g.download(), out.download(), Ch
 
 
-def test_half_small_complex_vs_fp64(ctx):
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_half_small_complex_vs_fp64(ctx, kernel):
     Sz, gSize, os_, planes = 65, 320, 8, 5
     iu, iv, co, data, C = small_problem(61, 4000, gSize, Sz, os_=os_, nPlanes=planes)
     vis = tcv.Visibilities(ctx, iu, iv, co, [Sz], gSize, C.size)
-    g, out, Ch = _half_run(ctx, vis, data, C, planes, Sz * Sz * os_ * os_, gSize)
+    g, out, Ch = _half_run(ctx, vis, data, C, planes, Sz * Sz * os_ * os_, gSize, kernel=kernel)
     g64 = O.grid_f64(data, iu, iv, co, C, gSize, Sz)
     e = rel_err(g, g64)
     print(f"fp16 grid err vs fp64 {e:.2e}")

@@ -143,7 +143,10 @@ def test_tiled_shapes_vs_oracle(ctx, tw, th):
     for tuning in (dict(tile_w=tw, tile_h=th),
                    dict(tile_w=tw, tile_h=th, slice_keys=1, phases=1),
                    dict(tile_w=tw, tile_h=th, chunk=64, slice_keys=256, phases=4),
-                   dict(tile_w=tw, tile_h=th, chunk=96, slice_keys=320, phases=16, xcd_split=1)):
+                   dict(tile_w=tw, tile_h=th, chunk=96, slice_keys=320, phases=16, xcd_split=1),
+                   dict(tile_w=tw, tile_h=th, kernel=2),
+                   dict(tile_w=tw, tile_h=th, chunk=96, slice_keys=320, phases=16, xcd_split=1,
+                        kernel=2)):
         g = gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning)
         assert rel_err(g, gref) <= TOL, tuning
         out = gpu_degrid(ctx, vis, gref, C, "tiled", tuning)
@@ -307,3 +310,31 @@ def test_copy_kernel_bit_exact(ctx):
         dst = ctx.empty(n, np.uint32)
         dst.copy_from(src)
         assert np.array_equal(dst.download(), a)
+
+
+@pytest.mark.parametrize("tw,th", [(64, 64), (32, 64), (16, 32)])
+def test_lds_gridder_bitwise_equal_to_register_gridder(ctx, tw, th):
+    """With every tile one work item (no C-slice ordering, items never split),
+    both gridders add each cell's terms in entry order with the same fused
+    operations, so their grids agree bit for bit; the degridders sum in a
+    different order and agree to rounding."""
+    Sz, gSize = 129, 400
+    iu, iv, co, data, C = small_problem(77 + tw, 2500, gSize, Sz, cluster=100)
+    vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
+    base = dict(tile_w=tw, tile_h=th, chunk=1 << 20, slice_keys=1, phases=1)
+    g1 = gpu_grid(ctx, vis, data, C, gSize, "tiled", dict(base, kernel=1))
+    g2 = gpu_grid(ctx, vis, data, C, gSize, "tiled", dict(base, kernel=2))
+    assert np.array_equal(g1, g2)
+    o1 = gpu_degrid(ctx, vis, g1, C, "tiled", dict(base, kernel=1))
+    o2 = gpu_degrid(ctx, vis, g1, C, "tiled", dict(base, kernel=2))
+    assert rel_err(o2, o1) <= TOL
+    assert rel_err(o2, O.degrid(g1, iu, iv, co, C, Sz)) <= TOL
+
+
+def test_tuning_kernel_field_validated(ctx):
+    Sz, gSize = 33, 128
+    iu, iv, co, data, C = small_problem(78, 50, gSize, Sz)
+    vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
+    with pytest.raises(tcv.TcvError):
+        tcv.Plan(ctx, vis, kernel=3)
+    assert tcv.Plan(ctx, vis, kernel=2).info()["kernel"] == 2

@@ -22,8 +22,8 @@ def own_ctx():
     c.close()
 
 
-def _job(c, seed=7, Sz=65, gSize=512, nVis=4000):
-    iu, iv, co, data, C = small_problem(seed, nVis, gSize, Sz, cluster=120)
+def _job(c, seed=7, Sz=65, gSize=512, nVis=4000, cluster=120):
+    iu, iv, co, data, C = small_problem(seed, nVis, gSize, Sz, cluster=cluster)
     vis = tcv.Visibilities(c, iu, iv, co, [Sz], gSize, C.size)
     return (iu, iv, co, data, C, Sz, gSize), vis
 
@@ -57,7 +57,7 @@ def test_autotune_candidates_and_cache(own_ctx):
 
 def test_autotune_builtin_set_grid_only(own_ctx):
     c = own_ctx
-    (iu, iv, co, data, C, Sz, gSize), vis = _job(c, seed=8, Sz=33, gSize=256, nVis=3000)
+    (iu, iv, co, data, C, Sz, gSize), vis = _job(c, seed=8, Sz=33, gSize=256, nVis=3000, cluster=60)
     d_data, d_C = c.to_device(data), c.to_device(C)
     g = c.zeros((gSize, gSize), np.complex64)
     best = tcv.autotune(c, vis, d_data, d_C, g, None, op=1)
