@@ -338,3 +338,25 @@ def test_tuning_kernel_field_validated(ctx):
     with pytest.raises(tcv.TcvError):
         tcv.Plan(ctx, vis, kernel=3)
     assert tcv.Plan(ctx, vis, kernel=2).info()["kernel"] == 2
+
+
+def test_repeat_runs_burnin_style(ctx):
+    """BurnIn-style check (attic/tConvolveBurnIn/Benchmark.cc:343-374): the
+    same work twice.  With every tile a single work item the tiled gridder
+    has no atomics and is bit-reproducible; with split work items the FP32
+    atomic flushes may round in a different order, within 1e-6 of the max."""
+    Sz, gSize = 129, 512
+    iu, iv, co, data, C = small_problem(91, 6000, gSize, Sz, cluster=150)
+    vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
+    for kernel in (1, 2):
+        det = dict(chunk=1 << 20, slice_keys=1, phases=1, kernel=kernel)
+        a = gpu_grid(ctx, vis, data, C, gSize, "tiled", det)
+        b = gpu_grid(ctx, vis, data, C, gSize, "tiled", det)
+        assert np.array_equal(a, b), kernel
+        split = dict(chunk=64, slice_keys=64, phases=8, kernel=kernel)
+        x = gpu_grid(ctx, vis, data, C, gSize, "tiled", split)
+        y = gpu_grid(ctx, vis, data, C, gSize, "tiled", split)
+        mismatches = int(np.count_nonzero(x != y))
+        print(f"kernel {kernel}: {mismatches} pixels differ between repeats (split items)")
+        assert rel_err(x, y) <= 1e-6
+        assert rel_err(x, a) <= TOL
