@@ -399,14 +399,24 @@ tcv_status tcv_range_errors(tcv_ctx *ctx, long long *count)
 // Defaults (a tuning field <= 0 means auto; xcd_split < 0 means auto):
 // tile width from the kernel width, 64-row tiles, 2048-entry work items, up to
 // 8192 C-slice keys, entries ordered by key in phases of about 4 MB of taps
-// (concurrently running workgroups share the phase's slices in L2), and, with
-// 8 or more phases, work items ordered XCD-major with one queue per XCD so
-// each XCD walks its own phases.
+// (concurrently running workgroups share the phase's slices in L2; fewer
+// phases when work items would get short), and, with 8 or more phases, work
+// items ordered XCD-major with one queue per XCD so each XCD walks its own
+// phases.  Kernels: kDefaultGridKernel / kDefaultDegridKernel.
 // C-slice phases: a power of two giving about 4 MB of taps per phase (one XCD's
-// L2), so that the slices one phase touches stay cached while its tiles run.
-static int default_phases(long long cSize, int sliceKeys)
+// L2), so that the slices one phase touches stay cached while its tiles run,
+// but no more than leaves about 64 entries per (tile, phase) work item on
+// average: every item pays for zeroing and flushing its tile.
+static int default_phases(long long cSize, int sliceKeys, const tcv_geometry *g, int maxS, int tw,
+                          int th)
 {
-    const long long target = std::max<long long>(1, cSize * 8 / (4LL << 20));
+    long long target = std::max<long long>(1, cSize * 8 / (4LL << 20));
+    if (g && tw > 0 && th > 0) {
+        const double perVis = (1.0 + (maxS - 1.0) / tw) * (1.0 + (maxS - 1.0) / th);
+        const double tiles = (double)((g->gSize + tw - 1) / tw) * ((g->gSize + th - 1) / th);
+        const long long byItems = (long long)(g->nVis * perVis / tiles / 64.0);
+        target = std::min(target, std::max<long long>(1, byItems));
+    }
     int ph = 1;
     while (ph * 2LL <= target && ph * 2 <= sliceKeys) ph *= 2;
     return ph;
@@ -421,9 +431,10 @@ static void auto_tuning(int maxS, const tcv_geometry *g, tcv_tuning *t)
     for (int k = 0; k < g->nPlanes; k++) minS = std::min(minS, g->sSize[k]);
     const long long slices = std::max<long long>(1, g->cSize / ((long long)minS * minS));
     if (t->slice_keys <= 0) t->slice_keys = (int)std::min<long long>(8192, slices);
-    if (t->phases <= 0) t->phases = default_phases(g->cSize, t->slice_keys);
+    if (t->phases <= 0) t->phases = default_phases(g->cSize, t->slice_keys, g, maxS, t->tile_w, t->tile_h);
     if (t->xcd_split < 0) t->xcd_split = (t->phases >= 8 && t->phases % 8 == 0) ? 1 : 0;
-    if (t->kernel <= 0) t->kernel = kDefaultTileKernel;
+    if (t->kernel <= 0) t->kernel = kDefaultGridKernel;
+    if (t->degrid_kernel <= 0) t->degrid_kernel = kDefaultDegridKernel;
 }
 
 static void free_plan(tcv_plan *p)
@@ -445,7 +456,7 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
     if (g->gSize > 65535) return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tiled path needs gSize <= 65535");
     if (g->cSize * 8 >= 0x7fffffffLL)
         return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tiled path needs a C table below 2 GiB");
-    tcv_tuning t = tuning ? *tuning : tcv_tuning{0, 0, 0, 0, 0, -1, 0};
+    tcv_tuning t = tuning ? *tuning : tcv_tuning{0, 0, 0, 0, 0, -1, 0, 0};
     if (!tuning && ctx) {
         auto it = ctx->tuned.find(tuning_class(g));
         if (it != ctx->tuned.end()) t = it->second;
@@ -462,9 +473,17 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
     sh.k2 = std::max(1, t.slice_keys);
     sh.phases = std::min(t.phases, sh.k2);
     sh.xcd = (t.xcd_split > 0 && sh.phases % 8 == 0) ? 1 : 0;
-    if (t.kernel != 1 && t.kernel != 2)
-        return tcv_fail(ctx, TCV_ERR_INVALID, "tuning.kernel must be 0, 1 or 2");
+    if (t.kernel < 1 || t.kernel > 3 || t.degrid_kernel < 1 || t.degrid_kernel > 3)
+        return tcv_fail(ctx, TCV_ERR_INVALID, "tuning.kernel and degrid_kernel must be 0..3");
     sh.kern = t.kernel;
+    sh.dkern = t.degrid_kernel;
+    // the pair kernels need kernel widths of at least 2 (a pair inside each C row)
+    int minS = maxS;
+    for (int k = 0; k < g->nPlanes; k++) minS = std::min(minS, g->sSize[k]);
+    if (minS < 2 || (sh.kern == 3 && !grid2_supported(sh.tw, sh.th))) {
+        if (sh.kern == 3) sh.kern = 1;
+    }
+    if (minS < 2 && sh.dkern == 3) sh.dkern = 2;
     sh.nTX = (g->gSize + sh.tw - 1) / sh.tw;
     sh.nTY = (g->gSize + sh.th - 1) / sh.th;
     sh.nTiles = sh.nTX * sh.nTY;
@@ -601,7 +620,8 @@ tcv_status tcv_plan_info(tcv_ctx *ctx, tcv_plan *p, tcv_tuning *t, long long *en
                          long long *items, long long *outOfRange)
 {
     if (!ctx || !p) return TCV_ERR_INVALID;
-    if (t) *t = tcv_tuning{p->sh.tw, p->sh.th, p->sh.chunk, p->sh.k2, p->sh.phases, p->sh.xcd, p->sh.kern};
+    if (t) *t = tcv_tuning{p->sh.tw, p->sh.th, p->sh.chunk, p->sh.k2, p->sh.phases, p->sh.xcd, p->sh.kern,
+                           p->sh.dkern};
     int meta[4] = {0, 0, 0, 0};   // items, queue head, entries, out of range
     TCV_HIP(ctx, hipMemcpyAsync(meta, p->meta, sizeof(meta), hipMemcpyDeviceToHost, ctx->stream));
     TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -725,7 +745,9 @@ tcv_status tcv_batch_create(tcv_ctx *ctx, int nJobs, const tcv_job *jobs, tcv_ba
             b->tw = J.plan->sh.tw;
             b->th = J.plan->sh.th;
             b->kern = J.plan->sh.kern;
-        } else if (J.plan->sh.tw != b->tw || J.plan->sh.th != b->th || J.plan->sh.kern != b->kern) {
+            b->dkern = J.plan->sh.dkern;
+        } else if (J.plan->sh.tw != b->tw || J.plan->sh.th != b->th || J.plan->sh.kern != b->kern ||
+                   J.plan->sh.dkern != b->dkern) {
             delete b;
             return tcv_fail(ctx, TCV_ERR_INVALID,
                             "tcv_batch_create: all plans of a batch need one tile shape and kernel");
@@ -871,7 +893,7 @@ static std::vector<tcv_tuning> builtin_candidates(const tcv_geometry *g)
     }
     const long long slices = std::max<long long>(1, g->cSize / ((long long)minS * minS));
     const int k2 = (int)std::min<long long>(8192, slices);
-    const int ph = default_phases(g->cSize, k2);
+    const int ph = default_phases(g->cSize, k2, g, maxS, 64, 64);
     std::vector<std::pair<int, int>> tiles;
     if (maxS >= 48) tiles = {{64, 64}, {64, 32}};
     else if (maxS >= 24) tiles = {{32, 64}, {32, 32}, {64, 32}};
@@ -881,11 +903,12 @@ static std::vector<tcv_tuning> builtin_candidates(const tcv_geometry *g)
     for (auto &t : tiles)
         for (int p : {ph / 2, ph, ph * 2}) {
             if (p < 1 || p > k2) continue;
-            for (int kern : {1, 2})
-                c.push_back(tcv_tuning{t.first, t.second, 2048, k2, p, p >= 8 ? 1 : 0, kern});
+            for (int kern : {1, 3})
+                for (int dk : {2, 3})
+                    c.push_back(tcv_tuning{t.first, t.second, 2048, k2, p, p >= 8 ? 1 : 0, kern, dk});
         }
-    c.push_back(tcv_tuning{tiles[0].first, tiles[0].second, 512, k2, ph, ph >= 8 ? 1 : 0, 0});
-    c.push_back(tcv_tuning{tiles[0].first, tiles[0].second, 2048, 1, 1, 0, 0});   // no ordering
+    c.push_back(tcv_tuning{tiles[0].first, tiles[0].second, 512, k2, ph, ph >= 8 ? 1 : 0, 0, 0});
+    c.push_back(tcv_tuning{tiles[0].first, tiles[0].second, 2048, 1, 1, 0, 0, 0});   // no ordering
     return c;
 }
 

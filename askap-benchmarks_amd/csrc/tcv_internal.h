@@ -103,12 +103,16 @@ struct JobDesc {
     int nVis;
 };
 
-// Tiled kernel used when a tuning leaves the choice open (1 registers, 2 LDS).
-constexpr int kDefaultTileKernel = 1;
+// Tiled kernels used when a tuning leaves the choice open.  Gridder: 1 tile in
+// registers, 2 tile in LDS, 3 registers with two columns per lane.  Degridder:
+// 1 registers, 2 LDS, 3 LDS with tap pairs.
+constexpr int kDefaultGridKernel = 1;
+constexpr int kDefaultDegridKernel = 2;
+bool grid2_supported(int tw, int th);
 
 struct PlanShape {
     int tw, th, chunk, k2, phases, xcd;
-    int kern;              // 1 register tile, 2 LDS tile
+    int kern, dkern;       // gridder / degridder kernel (codes as in tcv_tuning)
     int nTX, nTY, nTiles;
     int maxS;
     long long maxEntries;  // upper bound on visibility x tile pairs
@@ -147,7 +151,7 @@ void plan_bind(tcv_plan *p, char *block);
 
 struct tcv_batch {
     int nJobs = 0;
-    int tw = 0, th = 0, kern = kDefaultTileKernel;
+    int tw = 0, th = 0, kern = kDefaultGridKernel, dkern = kDefaultDegridKernel;
     bool half = false;
     std::vector<JobDesc> host;
     std::vector<tcv_plan *> plans;

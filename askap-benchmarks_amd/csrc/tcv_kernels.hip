@@ -518,6 +518,21 @@ struct Taps<true> {
     }
 };
 
+// Two consecutive taps (one 16-byte load for FP32, 8 bytes for FP16): the
+// vector memory path moves data per group of four lanes, so wider loads carry
+// twice the taps per unit of L1 throughput.
+template <bool HALF>
+__device__ __forceinline__ float4 load_pair(__amdgpu_buffer_rsrc_t r, unsigned voff)
+{
+    if constexpr (!HALF) {
+        return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0));
+    } else {
+        const uint2 u = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
+        const half2_t a = __builtin_bit_cast(half2_t, u.x), b = __builtin_bit_cast(half2_t, u.y);
+        return make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
+    }
+}
+
 __device__ __forceinline__ int entry_width(int w) { return w & 0xFFFF; }
 __device__ __forceinline__ int entry_plane(const int4 &en, const JobDesc &d)
 {
@@ -790,6 +805,134 @@ __global__ __launch_bounds__(NW * 64) void k_degrid_tiled(ItemSource<BATCH> src)
     }
 }
 
+// ---- tiled gridder, two columns per lane ------------------------------------
+// The register-tile gridder with each lane owning two adjacent tile columns,
+// so every C row segment arrives in 16-byte loads (two taps per lane).  Lane
+// l covers columns 2*(l % (TW/2)) and +1 of row l / (TW/2) in each group of
+// 128/TW rows.  A lane whose pair straddles the footprint's left or right
+// edge loads the pair that starts (or ends) on the edge tap and uses one half,
+// so every load stays inside the visibility's C slice.  Same per-cell
+// arithmetic and entry order as the kernels above.
+template <int TW, int R, int NW, int RB, bool HALF, bool BATCH>
+__global__ __launch_bounds__(NW * 64) void k_grid_tiled2(ItemSource<BATCH> src)
+{
+    constexpr int LPR = TW / 2;      // lanes per tile row
+    constexpr int RPI = 64 / LPR;    // tile rows per instruction
+    constexpr int TH = NW * R * RPI;
+    constexpr unsigned TB = Taps<HALF>::kBytes;
+    static_assert(R % RB == 0, "row block must divide the register rows");
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lx = 2 * (lane % LPR), ly = lane / LPR;
+    __shared__ int s_item;
+    const int nItems = src.total();
+    const int q0 = wave * R;
+
+    for (;;) {
+        if (threadIdx.x == 0) s_item = src.acquire();
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (item >= nItems) break;
+        JobDesc jd;
+        int local;
+        src.locate(item, jd, local);
+        const __amdgpu_buffer_rsrc_t rc = make_rsrc(jd.C, jd.cBytes);
+        const int4 *__restrict__ entries = jd.entries;
+        const int gSize = jd.gSize;
+        const int4 it = jd.items[local];
+        const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
+        const int Bx = txi * TW, By = tyi * TH;
+        const int e0 = it.y, e1 = it.y + it.z;
+
+        float ar0[R], ai0[R], ar1[R], ai1[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) { ar0[j] = ai0[j] = ar1[j] = ai1[j] = 0.f; }
+
+        int4 enA = entries[e0];
+        int4 enB = entries[min(e0 + 1, e1 - 1)];
+        float2 dA = vis_value<HALF>(jd, enA);
+        for (int e = e0; e < e1; ++e) {
+            const float2 dB = vis_value<HALF>(jd, enB);
+            const int4 enC = entries[min(e + 2, e1 - 1)];
+            const int u0 = enA.x & 0xFFFF, v0 = (int)((unsigned)enA.x >> 16);
+            const int c0 = enA.y, s = entry_width(enA.w);
+            const int cx = Bx + lx - u0;   // C column of the lane's first cell
+            // pair start and which half feeds which cell
+            const bool in0 = (unsigned)cx < (unsigned)s, in1 = (unsigned)(cx + 1) < (unsigned)s;
+            const int start = min(max(cx, 0), s - 2);
+            const bool sh = cx > start;    // right edge: the wanted tap is the pair's second
+            const unsigned lanev = (in0 || in1) ? (unsigned)(c0 + start) * TB : kOOB;
+            const int dy = By - v0;
+#pragma unroll
+            for (int jb = 0; jb < R; jb += RB) {
+                const int r0 = (q0 + jb) * RPI;
+                if (r0 + RB * RPI + dy > 0 && r0 + dy < s) {
+                    float4 cv[RB];
+#pragma unroll
+                    for (int k = 0; k < RB; ++k) {
+                        const int cy = r0 + k * RPI + ly + dy;
+                        const unsigned vo = ((unsigned)cy < (unsigned)s)
+                                                ? lanev + (unsigned)(cy * s) * TB : kOOB;
+                        cv[k] = load_pair<HALF>(rc, vo);
+                    }
+#pragma unroll
+                    for (int k = 0; k < RB; ++k) {
+                        // cell 0 takes tap 0 (tap 1 at the right edge); cell 1 takes
+                        // tap 1 (tap 0 when the pair starts one column early)
+                        float2 t0 = sh ? make_float2(cv[k].z, cv[k].w) : make_float2(cv[k].x, cv[k].y);
+                        float2 t1 = in0 ? make_float2(cv[k].z, cv[k].w) : make_float2(cv[k].x, cv[k].y);
+                        if (!in0) t0 = make_float2(0.f, 0.f);
+                        if (!in1) t1 = make_float2(0.f, 0.f);
+                        ar0[jb + k] = fmaf(dA.x, t0.x, fmaf(-dA.y, t0.y, ar0[jb + k]));
+                        ai0[jb + k] = fmaf(dA.x, t0.y, fmaf(dA.y, t0.x, ai0[jb + k]));
+                        ar1[jb + k] = fmaf(dA.x, t1.x, fmaf(-dA.y, t1.y, ar1[jb + k]));
+                        ai1[jb + k] = fmaf(dA.x, t1.y, fmaf(dA.y, t1.x, ai1[jb + k]));
+                    }
+                }
+            }
+            enA = enB;
+            dA = dB;
+            enB = enC;
+        }
+
+        const int col = Bx + lx;
+        const int rowsLeft = gSize - (By + q0 * RPI + ly);
+        float2 *__restrict__ g0 = jd.grid + (size_t)(By + q0 * RPI + ly) * gSize + col;
+        const bool c0ok = col < gSize, c1ok = col + 1 < gSize;
+        if (it.w) {
+            // the tile is this item's alone: plain read-modify-write
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                float2 *gp = g0 + (size_t)j * RPI * gSize;
+                const bool rok = j * RPI < rowsLeft;
+                if (rok && c0ok && (ar0[j] != 0.f || ai0[j] != 0.f)) {
+                    const float2 o = gp[0];
+                    gp[0] = make_float2(o.x + ar0[j], o.y + ai0[j]);
+                }
+                if (rok && c1ok && (ar1[j] != 0.f || ai1[j] != 0.f)) {
+                    const float2 o = gp[1];
+                    gp[1] = make_float2(o.x + ar1[j], o.y + ai1[j]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                float *fp = reinterpret_cast<float *>(g0 + (size_t)j * RPI * gSize);
+                const bool rok = j * RPI < rowsLeft;
+                if (rok && c0ok && (ar0[j] != 0.f || ai0[j] != 0.f)) {
+                    unsafeAtomicAdd(fp, ar0[j]);
+                    unsafeAtomicAdd(fp + 1, ai0[j]);
+                }
+                if (rok && c1ok && (ar1[j] != 0.f || ai1[j] != 0.f)) {
+                    unsafeAtomicAdd(fp + 2, ar1[j]);
+                    unsafeAtomicAdd(fp + 3, ai1[j]);
+                }
+            }
+        }
+    }
+}
+
 // ---- tiled gridder / degridder, LDS tile with dense lanes --------------------
 // The same work items as above, with the tile in LDS so that lanes map to the
 // points of each visibility's footprint on the tile rather than to tile
@@ -856,7 +999,7 @@ __global__ __launch_bounds__(256) void k_grid_lds(ItemSource<BATCH> src)
             const int n = w * ((rowHi - r0 + 3) >> 2);
             const float rw = 1.0f / (float)w;
             const int cBase = c0 + (By + r0 - v0) * s + (Bx + colLo - u0);
-            float2 *__restrict__ tBase = tile + r0 * TW + colLo;
+            float2 *tBase = tile + r0 * TW + colLo;
             // kLdsU points per lane per pass: their taps are loaded together
             for (int p0 = lane; p0 < n; p0 += 64 * kLdsU) {
                 float2 c[kLdsU];
@@ -964,6 +1107,106 @@ __global__ __launch_bounds__(256) void k_degrid_lds(ItemSource<BATCH> src)
                 for (int u = 0; u < kLdsU; ++u) {
                     sr = fmaf(g[u].x, c[u].x, fmaf(-g[u].y, c[u].y, sr));
                     si = fmaf(g[u].x, c[u].y, fmaf(g[u].y, c[u].x, si));
+                }
+            }
+            sr = wave_sum(sr);
+            si = wave_sum(si);
+            if constexpr (HALF) {
+                const float ps = jd.planeScale[entry_plane(en, jd)];
+                sr *= ps;
+                si *= ps;
+            }
+            if (lane == 0) {
+                float *op = reinterpret_cast<float *>(jd.out + en.z);
+                unsafeAtomicAdd(op, sr);
+                unsafeAtomicAdd(op + 1, si);
+            }
+        }
+    }
+}
+
+// Degridder with the LDS tile and tap pairs: lanes walk each entry's
+// footprint on the tile two columns at a time (16-byte C loads, two-cell LDS
+// reads).  The last pair of a row with an odd width uses one half; a pair that
+// would end past the C row starts one tap earlier instead.
+template <int TW, int TH, bool HALF, bool BATCH>
+__global__ __launch_bounds__(256) void k_degrid_lds2(ItemSource<BATCH> src)
+{
+    constexpr unsigned TB = Taps<HALF>::kBytes;
+    __shared__ float2 tile[TH * TW];
+    __shared__ int s_item;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nItems = src.total();
+
+    for (;;) {
+        if (threadIdx.x == 0) s_item = src.acquire();
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= nItems) break;
+        JobDesc jd;
+        int local;
+        src.locate(item, jd, local);
+        const __amdgpu_buffer_rsrc_t rc = make_rsrc(jd.C, jd.cBytes);
+        const int4 *__restrict__ entries = jd.entries;
+        const float2 *__restrict__ grid = jd.grid;
+        const int gSize = jd.gSize;
+        const int4 it = jd.items[local];
+        const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
+        const int Bx = txi * TW, By = tyi * TH;
+        for (int q = threadIdx.x; q < TH * TW; q += 256) {
+            const int row = By + q / TW, col = Bx + q % TW;
+            tile[q] = (row < gSize && col < gSize) ? grid[(size_t)row * gSize + col]
+                                                   : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+
+        for (int e = it.y + wave; e < it.y + it.z; e += 4) {
+            const int4 en = entries[e];
+            const int u0 = en.x & 0xFFFF, v0 = (int)((unsigned)en.x >> 16);
+            const int c0 = en.y, s = entry_width(en.w);
+            const int colLo = max(u0 - Bx, 0), colHi = min(u0 + s - Bx, TW);
+            const int rowLo = max(v0 - By, 0), rowHi = min(v0 + s - By, TH);
+            const int w = colHi - colLo, h = rowHi - rowLo;
+            if (w <= 0 || h <= 0) continue;
+            const int wp = (w + 1) >> 1;   // pairs per row
+            const int n = wp * h;
+            const float rw = 1.0f / (float)wp;
+            const int cxLo = Bx + colLo - u0;                 // C column of colLo
+            const int cRow = c0 + (By + rowLo - v0) * s;      // C offset of the first row
+            const float2 *tBase = tile + rowLo * TW + colLo;
+            float sr = 0.f, si = 0.f;
+            for (int p0 = lane; p0 < n; p0 += 64 * kLdsU) {
+                float4 c[kLdsU];
+                float2 g0[kLdsU], g1[kLdsU];
+#pragma unroll
+                for (int u = 0; u < kLdsU; ++u) {
+                    const int p = p0 + 64 * u;
+                    int rl, k;
+                    split_flat(p, rw, wp, rl, k);
+                    const int x = 2 * k;                      // first cell, from colLo
+                    const bool two = x + 1 < w;
+                    const int cx = cxLo + x;
+                    const int start = min(cx, s - 2);         // s >= 2 for this kernel
+                    c[u] = load_pair<HALF>(rc, p < n ? (unsigned)(cRow + rl * s + start) * TB : kOOB);
+                    const float2 *t = tBase + rl * TW + x;
+                    const float2 a = p < n ? t[0] : make_float2(0.f, 0.f);
+                    const float2 b = (p < n && two) ? t[1] : make_float2(0.f, 0.f);
+                    // the taps of cells x and x + 1 sit at (cx - start) and +1 in the pair
+                    if (cx > start) {   // pair starts one early: cell x uses the second tap
+                        g0[u] = make_float2(0.f, 0.f);
+                        g1[u] = a;
+                    } else {
+                        g0[u] = a;
+                        g1[u] = b;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kLdsU; ++u) {
+                    sr = fmaf(g0[u].x, c[u].x, fmaf(-g0[u].y, c[u].y, sr));
+                    si = fmaf(g0[u].x, c[u].y, fmaf(g0[u].y, c[u].x, si));
+                    sr = fmaf(g1[u].x, c[u].z, fmaf(-g1[u].y, c[u].w, sr));
+                    si = fmaf(g1[u].x, c[u].w, fmaf(g1[u].y, c[u].z, si));
                 }
             }
             sr = wave_sum(sr);
@@ -1247,6 +1490,7 @@ struct TileCfg {
     static constexpr int RPI = 64 / TW;
     static constexpr int RG = TH / (kNW * RPI);   // gridder rows groups per wave
     static constexpr int RD = TH / RPI;           // degridder row groups per wave
+    static constexpr int RG2 = TH / (kNW * (128 / TW));   // two-column gridder (0: not built)
 };
 
 int persistent_blocks(tcv_ctx *ctx, const void *kernel, int bs)
@@ -1277,6 +1521,10 @@ tcv_status grid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_kin
     using T = TileCfg<TW, TH>;
     auto kern = kern_kind == 2 ? k_grid_lds<TW, TH, HALF, BATCH>
                                : k_grid_tiled<TW, T::RG, kNW, (T::RG % kRB == 0 ? kRB : T::RG), HALF, BATCH>;
+    if constexpr (T::RG2 >= 1) {
+        if (kern_kind == 3)
+            kern = k_grid_tiled2<TW, T::RG2, kNW, (T::RG2 % kRB == 0 ? kRB : T::RG2), HALF, BATCH>;
+    }
     const int blocks = persistent_blocks(ctx, (const void *)kern, kNW * 64);
     ProfScope ps(ctx, BATCH ? "grid_batch" : (HALF ? "grid_tiled_half" : "grid_tiled"));
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kNW * 64), 0, ctx->stream, src);
@@ -1290,6 +1538,7 @@ tcv_status degrid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_k
     using T = TileCfg<TW, TH>;
     auto kern = kern_kind == 2 ? k_degrid_lds<TW, TH, HALF, BATCH>
                                : k_degrid_tiled<TW, T::RD, kNW, (T::RD % kRB == 0 ? kRB : T::RD), HALF, BATCH>;
+    if (kern_kind == 3) kern = k_degrid_lds2<TW, TH, HALF, BATCH>;
     const int blocks = persistent_blocks(ctx, (const void *)kern, kNW * 64);
     ProfScope ps(ctx, BATCH ? "degrid_batch" : (HALF ? "degrid_tiled_half" : "degrid_tiled"));
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kNW * 64), 0, ctx->stream, src);
@@ -1301,6 +1550,11 @@ tcv_status degrid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_k
 
 #define TCV_TILE_CONFIGS(X) \
     X(64, 64) X(64, 32) X(32, 64) X(32, 32) X(16, 64) X(16, 32) X(8, 64) X(8, 32)
+
+bool grid2_supported(int tw, int th)
+{
+    return th >= kNW * (128 / tw);
+}
 
 bool tiled_config_supported(int tw, int th)
 {
@@ -1358,8 +1612,8 @@ tcv_status launch_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *grid, co
     TCV_HIP(ctx, hipGetLastError());
 #define X(a, b)                                                                        \
     if (p->sh.tw == a && p->sh.th == b)                                                \
-        return half ? degrid_tiled_t<a, b, true, false>(ctx, src, p->sh.kern)          \
-                    : degrid_tiled_t<a, b, false, false>(ctx, src, p->sh.kern);
+        return half ? degrid_tiled_t<a, b, true, false>(ctx, src, p->sh.dkern)         \
+                    : degrid_tiled_t<a, b, false, false>(ctx, src, p->sh.dkern);
     TCV_TILE_CONFIGS(X)
 #undef X
     return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");
@@ -1379,7 +1633,7 @@ tcv_status launch_batch(tcv_ctx *ctx, tcv_batch *b, bool degrid)
     // batches run FP32 storage
 #define X(a, c)                                                                        \
     if (b->tw == a && b->th == c)                                                      \
-        return degrid ? degrid_tiled_t<a, c, false, true>(ctx, src, b->kern)           \
+        return degrid ? degrid_tiled_t<a, c, false, true>(ctx, src, b->dkern)          \
                       : grid_tiled_t<a, c, false, true>(ctx, src, b->kern);
     TCV_TILE_CONFIGS(X)
 #undef X

@@ -163,6 +163,10 @@ struct HogbomArgs {
 // Arithmetic is rounded exactly as the reference's float expressions.
 __global__ __launch_bounds__(256) void k_hogbom_step(HogbomArgs a, int i)
 {
+    // no fused multiply-add: each product and difference is rounded on its own,
+    // as in the reference's float expressions (plain operators under this
+    // pragma; helpers defined elsewhere would carry their own contract flags)
+#pragma clang fp contract(off)
     const unsigned long long kc = a.keys[i % 3];
     const unsigned long long kp = a.keys[3];
     const float peak = key_value(kc);
@@ -173,9 +177,9 @@ __global__ __launch_bounds__(256) void k_hogbom_step(HogbomArgs a, int i)
     const int diffx = rx - px, diffy = ry - px;   // as in the reference
     const int startx = max(0, rx - px), starty = max(0, ry - py);
     const int stopx = min(W - 1, rx + (PW - px - 1)), stopy = min(W - 1, ry + (PW - py - 1));
-    const float amp = __fmul_rn(a.gain, peak);
+    const float amp = a.gain * peak;
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-        a.model[pos].x = __fadd_rn(a.model[pos].x, __fmul_rn(peak, a.gain));
+        a.model[pos].x = a.model[pos].x + peak * a.gain;
         a.keys[(i + 2) % 3] = 0ull;
     }
     // rows split over blockIdx.y, columns over blockIdx.x: no per-pixel division
@@ -188,7 +192,7 @@ __global__ __launch_bounds__(256) void k_hogbom_step(HogbomArgs a, int i)
         for (int x = blockIdx.x * 256 + threadIdx.x; x < W; x += gridDim.x * 256) {
             float v = rrow[x];
             if (rowIn && x >= startx && x <= stopx) {
-                v = __fsub_rn(v, __fmul_rn(amp, prow[x]));
+                v = v - amp * prow[x];
                 rrow[x] = v;
             }
             const unsigned long long k = peak_key(v, (unsigned)y * (unsigned)W + (unsigned)x);

@@ -262,7 +262,7 @@ tcv_status tcv_server_create(tcv_ctx *ctx, const tcv_server_config *cfg, tcv_ser
     s->ctx = ctx;
     s->cfg = *cfg;
     s->tuning = tcv_tuning{cfg->tile_w, cfg->tile_h, cfg->chunk, cfg->slice_keys, cfg->phases, 0,
-                           kDefaultTileKernel};
+                           kDefaultGridKernel, kDefaultDegridKernel};
     if (s->tuning.tile_w <= 0) s->tuning.tile_w = 64;   // one tile shape for every batch
     if (s->tuning.tile_h <= 0) s->tuning.tile_h = 64;
     TCV_HIP(ctx, hipSetDevice(ctx->device));
@@ -281,6 +281,7 @@ tcv_status tcv_server_create(tcv_ctx *ctx, const tcv_server_config *cfg, tcv_ser
             b->tw = s->tuning.tile_w;
             b->th = s->tuning.tile_h;
             b->kern = s->tuning.kernel;
+            b->dkern = s->tuning.degrid_kernel;
             if (hipMalloc((void **)&b->d_jobs, sizeof(JobDesc) * maxJobs) != hipSuccess ||
                 hipMalloc((void **)&b->d_prefix, sizeof(int) * (maxJobs + 1)) != hipSuccess ||
                 hipMalloc((void **)&b->d_queue, 256) != hipSuccess)
@@ -331,8 +332,9 @@ tcv_status tcv_server_submit(tcv_server *s, const tcv_request *req, long long *i
     r->r.geom.sSize = r->sSize.data();
     tcv_status st = plan_setup(s->ctx, &r->r.geom, &s->tuning, &r->plan, &r->planBytes);
     if (st != TCV_OK) return st;
-    if (r->plan.sh.tw != s->tuning.tile_w || r->plan.sh.th != s->tuning.tile_h)
-        return TCV_ERR_UNSUPPORTED;
+    if (r->plan.sh.tw != s->tuning.tile_w || r->plan.sh.th != s->tuning.tile_h ||
+        r->plan.sh.kern != s->tuning.kernel || r->plan.sh.dkern != s->tuning.degrid_kernel)
+        return TCV_ERR_UNSUPPORTED;   // every job of a batch runs the same kernels
     r->resBytes = req->op == 0 ? sizeof(float2) * (size_t)G.gSize * G.gSize
                                : sizeof(float2) * (size_t)G.nVis;
     long long pts = 0;

@@ -43,7 +43,8 @@ def alg_bytes(points, nVis, gSize, op):
 
 
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_grid_degrid_tiled_spectral.md")
-TUNING_KEYS = ("tile_w", "tile_h", "chunk", "slice_keys", "phases", "xcd_split", "kernel")
+TUNING_KEYS = ("tile_w", "tile_h", "chunk", "slice_keys", "phases", "xcd_split", "kernel",
+               "degrid_kernel")
 
 
 def measured_traffic(kernel, info):
@@ -260,7 +261,10 @@ def main():
     ap.add_argument("--slice-keys", type=int, default=0)
     ap.add_argument("--phases", type=int, default=0)
     ap.add_argument("--xcd-split", type=int, default=-1)
-    ap.add_argument("--kernel", type=int, default=0, help="1 register tile, 2 LDS tile, 0 auto")
+    ap.add_argument("--kernel", type=int, default=0,
+                    help="gridder: 1 register tile, 2 LDS tile, 3 register tile with tap pairs; 0 auto")
+    ap.add_argument("--degrid-kernel", type=int, default=0,
+                    help="degridder: 1 register tile, 2 LDS tile, 3 LDS tile with tap pairs; 0 auto")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -288,7 +292,7 @@ def main():
     ctx = tcv.Context(0)
     tuning = dict(tile_w=args.tile_w, tile_h=args.tile_h, chunk=args.chunk,
                   slice_keys=args.slice_keys, phases=args.phases, xcd_split=args.xcd_split,
-                  kernel=args.kernel)
+                  kernel=args.kernel, degrid_kernel=args.degrid_kernel)
     job = tcv.GridJob(ctx, prob, tuning=tuning)
     assert job.nBad == 0, f"{job.nBad} visibilities out of range"
     points = job.points

@@ -191,8 +191,11 @@ typedef struct {
     int xcd_split;   /* 1: order work items XCD-major (phases % 8 == 0), one queue per
                         XCD so each XCD's L2 holds its share of a phase's C slices;
                         0 = one queue; -1 = auto */
-    int kernel;      /* 1: tile in registers, lanes = tile columns; 2: tile in LDS,
-                        lanes walk each footprint's points densely; 0 = auto */
+    int kernel;      /* gridder: 1 tile in registers, lanes = tile columns; 2 tile in
+                        LDS, lanes walk each footprint's points densely; 3 registers,
+                        two columns per lane (16-byte C loads); 0 = auto */
+    int degrid_kernel; /* degridder: 1 tile in registers; 2 tile in LDS, dense lanes;
+                        3 LDS with tap pairs (16-byte C loads); 0 = auto */
 } tcv_tuning;
 
 typedef struct tcv_plan tcv_plan;

@@ -97,8 +97,10 @@ def test_device_initC_matches_host_taps(ctx):
     assert np.abs(d.astype(np.complex128)).sum() == pytest.approx(33 * 64, rel=1e-6)
 
 
-def test_batch_lds_kernel_vs_oracle(ctx):
-    bj = BatchJobs(ctx, small_specs(4, seed=21), tuning=dict(kernel=2))
+@pytest.mark.parametrize("kernels", [(2, 2), (3, 3), (1, 3)])
+def test_batch_other_kernels_vs_oracle(ctx, kernels):
+    bj = BatchJobs(ctx, small_specs(4, seed=21), tuning=dict(kernel=kernels[0],
+                                                              degrid_kernel=kernels[1]))
     bj.zero_grids()
     bj.run_batched(graph=True)
     for j in range(len(bj.jobs)):

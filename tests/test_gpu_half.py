@@ -18,10 +18,11 @@ HALF_TOL = 2e-3
 
 
 def _half_run(ctx, vis, data, C, nPlanes, planeTaps, gSize, planeStart=None, kernel=0):
+    """kernel: the same code for the gridder and the degridder (0 = auto)."""
     d_C, d_data = ctx.to_device(C), ctx.to_device(data)
     Ch = tcv.HalfTable(ctx, d_C, nPlanes, planeTaps, planeStart)
     dh = tcv.HalfVis(ctx, d_data)
-    plan = tcv.Plan(ctx, vis, kernel=kernel)
+    plan = tcv.Plan(ctx, vis, kernel=kernel, degrid_kernel=kernel)
     g = ctx.zeros((gSize, gSize), np.complex64)
     plan.grid_half(dh, Ch, g)
     out = ctx.empty(vis.nVis, np.complex64)
@@ -29,7 +30,7 @@ def _half_run(ctx, vis, data, C, nPlanes, planeTaps, gSize, planeStart=None, ker
     return g.download(), out.download(), Ch
 
 
-@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("kernel", [1, 2, 3])
 def test_half_small_complex_vs_fp64(ctx, kernel):
     Sz, gSize, os_, planes = 65, 320, 8, 5
     iu, iv, co, data, C = small_problem(61, 4000, gSize, Sz, os_=os_, nPlanes=planes)

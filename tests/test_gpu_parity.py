@@ -144,9 +144,12 @@ def test_tiled_shapes_vs_oracle(ctx, tw, th):
                    dict(tile_w=tw, tile_h=th, slice_keys=1, phases=1),
                    dict(tile_w=tw, tile_h=th, chunk=64, slice_keys=256, phases=4),
                    dict(tile_w=tw, tile_h=th, chunk=96, slice_keys=320, phases=16, xcd_split=1),
-                   dict(tile_w=tw, tile_h=th, kernel=2),
+                   dict(tile_w=tw, tile_h=th, kernel=2, degrid_kernel=2),
                    dict(tile_w=tw, tile_h=th, chunk=96, slice_keys=320, phases=16, xcd_split=1,
-                        kernel=2)):
+                        kernel=2, degrid_kernel=1),
+                   dict(tile_w=tw, tile_h=th, kernel=3, degrid_kernel=3),
+                   dict(tile_w=tw, tile_h=th, chunk=96, slice_keys=320, phases=16, xcd_split=1,
+                        kernel=3, degrid_kernel=3)):
         g = gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning)
         assert rel_err(g, gref) <= TOL, tuning
         out = gpu_degrid(ctx, vis, gref, C, "tiled", tuning)
@@ -312,23 +315,60 @@ def test_copy_kernel_bit_exact(ctx):
         assert np.array_equal(dst.download(), a)
 
 
+def exact_problem(seed, nVis, gSize, Sz, cluster):
+    """Inputs whose every partial sum is exact in FP32: C taps are multiples
+    of 1/4 in [-1, 1], data and grid values small integers.  Results then do
+    not depend on summation order, so different kernels, plans and the oracle
+    must agree bit for bit."""
+    iu, iv, co, _, C = small_problem(seed, nVis, gSize, Sz, cluster=cluster)
+    rng = np.random.default_rng(seed + 1)
+    C = (rng.integers(-4, 5, C.size) / 4 + 1j * rng.integers(-4, 5, C.size) / 4).astype(np.complex64)
+    data = (rng.integers(-3, 4, nVis) + 1j * rng.integers(-3, 4, nVis)).astype(np.complex64)
+    grid = (rng.integers(-2, 3, (gSize, gSize)) + 1j * rng.integers(-2, 3, (gSize, gSize))).astype(np.complex64)
+    return iu, iv, co, data, C, grid
+
+
 @pytest.mark.parametrize("tw,th", [(64, 64), (32, 64), (16, 32)])
-def test_lds_gridder_bitwise_equal_to_register_gridder(ctx, tw, th):
-    """With every tile one work item (no C-slice ordering, items never split),
-    both gridders add each cell's terms in entry order with the same fused
-    operations, so their grids agree bit for bit; the degridders sum in a
-    different order and agree to rounding."""
+def test_lds_and_register_kernels_bitwise_on_exact_inputs(ctx, tw, th):
+    """On order-independent (exactly representable) sums the register-tile and
+    LDS-tile kernels, with split or whole work items, equal the oracle bit for
+    bit: every visibility reaches every cell of its footprint exactly once."""
     Sz, gSize = 129, 400
-    iu, iv, co, data, C = small_problem(77 + tw, 2500, gSize, Sz, cluster=100)
+    iu, iv, co, data, C, G = exact_problem(77 + tw, 2500, gSize, Sz, cluster=100)
     vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
-    base = dict(tile_w=tw, tile_h=th, chunk=1 << 20, slice_keys=1, phases=1)
-    g1 = gpu_grid(ctx, vis, data, C, gSize, "tiled", dict(base, kernel=1))
-    g2 = gpu_grid(ctx, vis, data, C, gSize, "tiled", dict(base, kernel=2))
-    assert np.array_equal(g1, g2)
-    o1 = gpu_degrid(ctx, vis, g1, C, "tiled", dict(base, kernel=1))
-    o2 = gpu_degrid(ctx, vis, g1, C, "tiled", dict(base, kernel=2))
-    assert rel_err(o2, o1) <= TOL
-    assert rel_err(o2, O.degrid(g1, iu, iv, co, C, Sz)) <= TOL
+    gref = O.grid(data, iu, iv, co, C, gSize, Sz)
+    oref = O.degrid(G, iu, iv, co, C, Sz)
+    for kernel in (1, 2, 3):
+        for t in (dict(chunk=1 << 20, slice_keys=1, phases=1),
+                  dict(chunk=64, slice_keys=64, phases=8, xcd_split=1)):
+            tuning = dict(t, tile_w=tw, tile_h=th, kernel=kernel, degrid_kernel=kernel)
+            assert np.array_equal(gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning), gref), tuning
+            assert np.array_equal(gpu_degrid(ctx, vis, G, C, "tiled", tuning), oref), tuning
+
+
+@pytest.mark.parametrize("Sz", [3, 5, 7, 33, 65, 129])
+def test_pair_kernels_every_width_exact(ctx, Sz):
+    """The two-tap kernels at every footprint parity and width, including
+    footprints that start or end on a tile edge: bit-exact on exact inputs."""
+    gSize = 256
+    iu, iv, co, data, C, G = exact_problem(300 + Sz, 1500, gSize, Sz, cluster=60)
+    # add footprints at both grid edges and at every offset around a tile edge
+    half = Sz // 2
+    extra_u = np.array([half, gSize - half - 1] + [64 + d for d in range(-Sz, 2)], np.int32)
+    extra_u = np.clip(extra_u, half, gSize - half - 1)
+    extra_v = np.clip(np.arange(extra_u.size, dtype=np.int32) * 7 % (gSize - Sz + 1), 0, gSize - Sz)
+    iu = np.concatenate([iu, extra_u])
+    iv = np.concatenate([iv, extra_v.astype(np.int32)])
+    co = np.concatenate([co, co[:extra_u.size]])
+    rng = np.random.default_rng(Sz)
+    data = np.concatenate([data, (rng.integers(-3, 4, extra_u.size) + 0j).astype(np.complex64)])
+    vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
+    gref = O.grid(data, iu, iv, co, C, gSize, Sz)
+    oref = O.degrid(G, iu, iv, co, C, Sz)
+    for tw, th in ((64, 64), (32, 64), (16, 32), (8, 64)):
+        tuning = dict(tile_w=tw, tile_h=th, kernel=3, degrid_kernel=3, phases=4, slice_keys=16, chunk=32)
+        assert np.array_equal(gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning), gref), (tw, th)
+        assert np.array_equal(gpu_degrid(ctx, vis, G, C, "tiled", tuning), oref), (tw, th)
 
 
 def test_tuning_kernel_field_validated(ctx):
@@ -336,27 +376,38 @@ def test_tuning_kernel_field_validated(ctx):
     iu, iv, co, data, C = small_problem(78, 50, gSize, Sz)
     vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
     with pytest.raises(tcv.TcvError):
-        tcv.Plan(ctx, vis, kernel=3)
-    assert tcv.Plan(ctx, vis, kernel=2).info()["kernel"] == 2
+        tcv.Plan(ctx, vis, kernel=4)
+    with pytest.raises(tcv.TcvError):
+        tcv.Plan(ctx, vis, degrid_kernel=-2)
+    info = tcv.Plan(ctx, vis, kernel=2, degrid_kernel=3).info()
+    assert (info["kernel"], info["degrid_kernel"]) == (2, 3)
+    # the two-column gridder needs 4 waves x 128/tile_w rows: (8, 32) falls back
+    assert tcv.Plan(ctx, vis, tile_w=8, tile_h=32, kernel=3).info()["kernel"] == 1
 
 
 def test_repeat_runs_burnin_style(ctx):
     """BurnIn-style check (attic/tConvolveBurnIn/Benchmark.cc:343-374): the
-    same work twice.  With every tile a single work item the tiled gridder
-    has no atomics and is bit-reproducible; with split work items the FP32
-    atomic flushes may round in a different order, within 1e-6 of the max."""
+    same work twice on one plan.  With every tile a single work item the tiled
+    gridder has no atomics and is bit-reproducible; with split work items the
+    FP32 atomic flushes may round in a different order (within 1e-6 of the
+    max), and so may a rebuilt plan (its binning order is not fixed)."""
     Sz, gSize = 129, 512
     iu, iv, co, data, C = small_problem(91, 6000, gSize, Sz, cluster=150)
     vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
-    for kernel in (1, 2):
-        det = dict(chunk=1 << 20, slice_keys=1, phases=1, kernel=kernel)
-        a = gpu_grid(ctx, vis, data, C, gSize, "tiled", det)
-        b = gpu_grid(ctx, vis, data, C, gSize, "tiled", det)
+    d_data, d_C = ctx.to_device(data), ctx.to_device(C)
+
+    def run(plan):
+        g = ctx.zeros((gSize, gSize), np.complex64)
+        plan.grid(d_data, d_C, g)
+        return g.download()
+
+    for kernel in (1, 2, 3):
+        det = tcv.Plan(ctx, vis, chunk=1 << 20, slice_keys=1, phases=1, kernel=kernel)
+        a, b = run(det), run(det)
         assert np.array_equal(a, b), kernel
-        split = dict(chunk=64, slice_keys=64, phases=8, kernel=kernel)
-        x = gpu_grid(ctx, vis, data, C, gSize, "tiled", split)
-        y = gpu_grid(ctx, vis, data, C, gSize, "tiled", split)
-        mismatches = int(np.count_nonzero(x != y))
-        print(f"kernel {kernel}: {mismatches} pixels differ between repeats (split items)")
+        split = tcv.Plan(ctx, vis, chunk=64, slice_keys=64, phases=8, kernel=kernel,
+                         degrid_kernel=kernel)
+        x, y = run(split), run(split)
+        print(f"kernel {kernel}: {int(np.count_nonzero(x != y))} pixels differ between repeats")
         assert rel_err(x, y) <= 1e-6
         assert rel_err(x, a) <= TOL
