@@ -106,8 +106,8 @@ struct JobDesc {
 // Tiled kernels used when a tuning leaves the choice open.  Gridder: 1 tile in
 // registers, 2 tile in LDS, 3 registers with two columns per lane.  Degridder:
 // 1 registers, 2 LDS, 3 LDS with tap pairs.
-constexpr int kDefaultGridKernel = 1;
-constexpr int kDefaultDegridKernel = 2;
+constexpr int kDefaultGridKernel = 3;
+constexpr int kDefaultDegridKernel = 3;
 bool grid2_supported(int tw, int th);
 
 struct PlanShape {

@@ -319,12 +319,12 @@ tcv_status tcv_server_destroy(tcv_server *s)
 
 double tcv_server_clock(tcv_server *s) { return s ? s->now() : 0.0; }
 
-tcv_status tcv_server_submit(tcv_server *s, const tcv_request *req, long long *id)
+// Validates a request and sizes its plan and result (no device work).
+static tcv_status prepare(tcv_server *s, const tcv_request *req, Req *r)
 {
-    if (!s || !req || !id || (req->op != 0 && req->op != 1) || !req->d_C ||
+    if (!req || (req->op != 0 && req->op != 1) || !req->d_C ||
         (req->op == 0 && !req->d_data) || (req->op == 1 && !req->d_grid))
         return TCV_ERR_INVALID;
-    auto r = std::make_unique<Req>();
     r->r = *req;
     const tcv_geometry &G = req->geom;
     if (G.nPlanes < 1 || !G.sSize) return TCV_ERR_INVALID;
@@ -340,6 +340,25 @@ tcv_status tcv_server_submit(tcv_server *s, const tcv_request *req, long long *i
     long long pts = 0;
     for (int k = 0; k < G.nPlanes; k++) pts = std::max<long long>(pts, r->sSize[k]);
     r->points = (long long)G.nVis * pts * pts;   // upper bound when widths vary
+    return TCV_OK;
+}
+
+tcv_status tcv_server_footprint(tcv_server *s, const tcv_request *req, size_t *bytes)
+{
+    if (!s || !bytes) return TCV_ERR_INVALID;
+    Req r;
+    tcv_status st = prepare(s, req, &r);
+    if (st != TCV_OK) return st;
+    *bytes = tcv_detail::Arena::round(r.planBytes) + tcv_detail::Arena::round(r.resBytes);
+    return TCV_OK;
+}
+
+tcv_status tcv_server_submit(tcv_server *s, const tcv_request *req, long long *id)
+{
+    if (!s || !id) return TCV_ERR_INVALID;
+    auto r = std::make_unique<Req>();
+    tcv_status st = prepare(s, req, r.get());
+    if (st != TCV_OK) return st;
     std::lock_guard<std::mutex> lk(s->mu);
     if (!s->arena.fits_ever(tcv_detail::Arena::round(r->planBytes) + r->resBytes)) {
         s->err = "request needs " + std::to_string(r->planBytes + r->resBytes) +

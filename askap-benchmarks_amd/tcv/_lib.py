@@ -116,6 +116,7 @@ SIGS = {
                          ct.POINTER(Tuning), PD]),
     "tcv_server_create": (I, [P, ct.POINTER(ServerConfig), ct.POINTER(P)]),
     "tcv_server_destroy": (I, [P]),
+    "tcv_server_footprint": (I, [P, ct.POINTER(Request), ct.POINTER(ct.c_size_t)]),
     "tcv_server_submit": (I, [P, ct.POINTER(Request), PLL]),
     "tcv_server_cancel": (I, [P, LL]),
     "tcv_server_wait": (I, [P, LL, D, ct.POINTER(Result)]),

@@ -414,6 +414,15 @@ class Server:
         self._keep[rid.value] = vis   # the geometry's device arrays must outlive the request
         return rid.value
 
+    def footprint(self, op: int, vis: Visibilities, C: DeviceArray, data: DeviceArray = None,
+                  grid: DeviceArray = None) -> int:
+        """Arena bytes the request would hold while it runs (plan + result)."""
+        req = Request(op, vis.geometry(), data.ptr if data is not None else None, C.ptr,
+                      grid.ptr if grid is not None else None)
+        b = ct.c_size_t()
+        self._check(lib().tcv_server_footprint(self.h, ct.byref(req), ct.byref(b)), "server_footprint")
+        return b.value
+
     def cancel(self, rid: int):
         self._check(lib().tcv_server_cancel(self.h, rid), "server_cancel")
 

@@ -350,6 +350,9 @@ typedef struct {
 tcv_status tcv_server_create(tcv_ctx *ctx, const tcv_server_config *cfg, tcv_server **srv);
 /* Stops the scheduler after the batches on the device finish; frees the arena. */
 tcv_status tcv_server_destroy(tcv_server *srv);
+/* Arena bytes a request would hold while it runs (its plan plus its result),
+ * the amount admission waits for.  Validates the request; enqueues nothing. */
+tcv_status tcv_server_footprint(tcv_server *srv, const tcv_request *req, size_t *bytes);
 /* Thread-safe.  TCV_ERR_BUDGET when the request can never fit the budget. */
 tcv_status tcv_server_submit(tcv_server *srv, const tcv_request *req, long long *id);
 /* Cancels a request that is still queued (TCV_ERR_UNSUPPORTED once admitted). */

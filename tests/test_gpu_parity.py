@@ -378,7 +378,7 @@ def test_tuning_kernel_field_validated(ctx):
     with pytest.raises(tcv.TcvError):
         tcv.Plan(ctx, vis, kernel=4)
     with pytest.raises(tcv.TcvError):
-        tcv.Plan(ctx, vis, degrid_kernel=-2)
+        tcv.Plan(ctx, vis, degrid_kernel=7)
     info = tcv.Plan(ctx, vis, kernel=2, degrid_kernel=3).info()
     assert (info["kernel"], info["degrid_kernel"]) == (2, 3)
     # the two-column gridder needs 4 waves x 128/tile_w rows: (8, 32) falls back

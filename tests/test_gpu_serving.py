@@ -87,26 +87,11 @@ def test_budget_refuses_requests_that_never_fit(ctx, templates):
 
 def test_admission_waits_for_memory_and_cancel(ctx, templates):
     big = max(templates.jobs, key=lambda j: j["spec"]["gSize"])
-    # find one request's footprint by running it in a roomy server
-    srv = tcv.Server(ctx, 8 * GB, max_inflight=1)
-    rid = submit(srv, big, 0)
-    srv.wait(rid)
-    srv.release(rid)
-    srv.close()
     # an arena that holds one such request at a time: the next ones wait
     probe = tcv.Server(ctx, 8 * GB, max_inflight=1)
-    rid = submit(probe, big, 0)
-    used = 0
-    while True:
-        st = probe.stats()
-        used = max(used, st["arena_used"])
-        if st["queued"] == 0 and st["running"] == 0:
-            break
-        time.sleep(0.0005)
-    probe.wait(rid)
-    probe.release(rid)
+    used = probe.footprint(0, big["vis"], big["C"], data=big["data"])
     probe.close()
-    assert used > 0
+    assert used > big["spec"]["gSize"] ** 2 * 8
     srv = tcv.Server(ctx, int(used * 1.5), max_inflight=2)
     ids = [submit(srv, big, 0) for _ in range(4)]
     victim = ids[-1]
