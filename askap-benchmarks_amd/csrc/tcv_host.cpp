@@ -433,6 +433,7 @@ static void auto_tuning(int maxS, const tcv_geometry *g, tcv_tuning *t)
     if (t->slice_keys <= 0) t->slice_keys = (int)std::min<long long>(8192, slices);
     if (t->phases <= 0) t->phases = default_phases(g->cSize, t->slice_keys, g, maxS, t->tile_w, t->tile_h);
     if (t->xcd_split < 0) t->xcd_split = (t->phases >= 8 && t->phases % 8 == 0) ? 1 : 0;
+    if (t->grid_phases <= 0) t->grid_phases = t->phases >= 16 ? t->phases / 2 : t->phases;
     if (t->kernel <= 0) t->kernel = kDefaultGridKernel;
     if (t->degrid_kernel <= 0) t->degrid_kernel = kDefaultDegridKernel;
 }
@@ -456,7 +457,7 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
     if (g->gSize > 65535) return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tiled path needs gSize <= 65535");
     if (g->cSize * 8 >= 0x7fffffffLL)
         return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tiled path needs a C table below 2 GiB");
-    tcv_tuning t = tuning ? *tuning : tcv_tuning{0, 0, 0, 0, 0, -1, 0, 0};
+    tcv_tuning t = tuning ? *tuning : tcv_tuning{0, 0, 0, 0, 0, -1, 0, 0, 0};
     if (!tuning && ctx) {
         auto it = ctx->tuned.find(tuning_class(g));
         if (it != ctx->tuned.end()) t = it->second;
@@ -473,6 +474,8 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
     sh.k2 = std::max(1, t.slice_keys);
     sh.phases = std::min(t.phases, sh.k2);
     sh.xcd = (t.xcd_split > 0 && sh.phases % 8 == 0) ? 1 : 0;
+    sh.phasesG = std::min(t.grid_phases, sh.k2);
+    sh.xcdG = (t.xcd_split > 0 && sh.phasesG % 8 == 0) ? 1 : 0;
     if (t.kernel < 1 || t.kernel > 3 || t.degrid_kernel < 1 || t.degrid_kernel > 3)
         return tcv_fail(ctx, TCV_ERR_INVALID, "tuning.kernel and degrid_kernel must be 0..3");
     sh.kern = t.kernel;
@@ -494,13 +497,16 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
     if (sh.maxEntries >= 0x7fffffffLL || sh.nKeys >= 0x7fffffffLL)
         return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "plan too large for 32-bit entry indices");
     p->maxItems = sh.maxEntries / sh.chunk + (long long)sh.phases * sh.nTiles + 1;
+    const bool twoLists = sh.phasesG != sh.phases || sh.xcdG != sh.xcd;
+    p->maxItemsG = twoLists ? sh.maxEntries / sh.chunk + (long long)sh.phasesG * sh.nTiles + 1 : 0;
+    const long long slotsG = (long long)sh.phasesG * sh.nTiles;
     p->nVis = g->nVis;
     p->gSize = g->gSize;
     p->nPlanes = g->nPlanes;
     p->cSize = g->cSize;
     p->sSizeHost.assign(g->sSize, g->sSize + g->nPlanes);
     const long long slots = (long long)sh.phases * sh.nTiles;
-    p->scanTmpLen = std::max(scan_tmp_len(sh.nKeys), scan_tmp_len(slots));
+    p->scanTmpLen = std::max(scan_tmp_len(sh.nKeys), std::max(scan_tmp_len(slots), scan_tmp_len(slotsG)));
     size_t n = 0;
     n += align256(sizeof(int) * g->nPlanes);
     n += align256(sizeof(int4) * sh.maxEntries);
@@ -509,6 +515,11 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
     n += 2 * align256(sizeof(int) * (slots + 1));
     n += align256(sizeof(int) * 32);
     n += align256(sizeof(int) * p->scanTmpLen);
+    if (twoLists) {
+        n += align256(sizeof(int4) * p->maxItemsG);
+        n += 2 * align256(sizeof(int) * (slotsG + 1));
+        n += align256(sizeof(int) * 32);
+    }
     *bytes = n;
     return TCV_OK;
 }
@@ -533,6 +544,18 @@ void plan_bind(tcv_plan *p, char *block)
     p->itemStart = (int *)take(sizeof(int) * (slots + 1));
     p->meta = (int *)take(sizeof(int) * 32);
     p->scanTmp = (int *)take(sizeof(int) * p->scanTmpLen);
+    if (p->maxItemsG) {
+        const long long slotsG = (long long)sh.phasesG * sh.nTiles;
+        p->itemsG = (int4 *)take(sizeof(int4) * p->maxItemsG);
+        p->itemCountG = (int *)take(sizeof(int) * (slotsG + 1));
+        p->itemStartG = (int *)take(sizeof(int) * (slotsG + 1));
+        p->metaG = (int *)take(sizeof(int) * 32);
+    } else {
+        p->itemsG = p->items;
+        p->itemCountG = p->itemCount;
+        p->itemStartG = p->itemStart;
+        p->metaG = p->meta;
+    }
     p->block = block;
 }
 
@@ -621,7 +644,7 @@ tcv_status tcv_plan_info(tcv_ctx *ctx, tcv_plan *p, tcv_tuning *t, long long *en
 {
     if (!ctx || !p) return TCV_ERR_INVALID;
     if (t) *t = tcv_tuning{p->sh.tw, p->sh.th, p->sh.chunk, p->sh.k2, p->sh.phases, p->sh.xcd, p->sh.kern,
-                           p->sh.dkern};
+                           p->sh.dkern, p->sh.phasesG};
     int meta[4] = {0, 0, 0, 0};   // items, queue head, entries, out of range
     TCV_HIP(ctx, hipMemcpyAsync(meta, p->meta, sizeof(meta), hipMemcpyDeviceToHost, ctx->stream));
     TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -753,20 +776,24 @@ tcv_status tcv_batch_create(tcv_ctx *ctx, int nJobs, const tcv_job *jobs, tcv_ba
                             "tcv_batch_create: all plans of a batch need one tile shape and kernel");
         }
         b->plans.push_back(J.plan);
-        b->host.push_back(make_job_desc(J.plan, J.d_data, J.d_C, (float2 *)J.d_grid, (float2 *)J.d_out, nullptr));
+        b->host.push_back(make_job_desc(J.plan, true, J.d_data, J.d_C, (float2 *)J.d_grid, (float2 *)J.d_out, nullptr));
+        b->hostD.push_back(make_job_desc(J.plan, false, J.d_data, J.d_C, (float2 *)J.d_grid, (float2 *)J.d_out, nullptr));
     }
     auto fail = [&](const char *m) {
         if (b->d_jobs) (void)hipFree(b->d_jobs);
+        if (b->d_jobsD) (void)hipFree(b->d_jobsD);
         if (b->d_prefix) (void)hipFree(b->d_prefix);
         if (b->d_queue) (void)hipFree(b->d_queue);
         delete b;
         return tcv_fail(ctx, TCV_ERR_NOMEM, m);
     };
     if (hipMalloc((void **)&b->d_jobs, sizeof(JobDesc) * nJobs) != hipSuccess ||
+        hipMalloc((void **)&b->d_jobsD, sizeof(JobDesc) * nJobs) != hipSuccess ||
         hipMalloc((void **)&b->d_prefix, sizeof(int) * (nJobs + 1)) != hipSuccess ||
         hipMalloc((void **)&b->d_queue, 256) != hipSuccess)
         return fail("tcv_batch_create: allocation failed");
-    if (hipMemcpy(b->d_jobs, b->host.data(), sizeof(JobDesc) * nJobs, hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpy(b->d_jobs, b->host.data(), sizeof(JobDesc) * nJobs, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_jobsD, b->hostD.data(), sizeof(JobDesc) * nJobs, hipMemcpyHostToDevice) != hipSuccess)
         return fail("tcv_batch_create: upload failed");
     *out = b;
     return TCV_OK;
@@ -779,6 +806,7 @@ tcv_status tcv_batch_destroy(tcv_ctx *ctx, tcv_batch *b)
     TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     (void)hipFree(b->d_jobs);
+    (void)hipFree(b->d_jobsD);
     (void)hipFree(b->d_prefix);
     (void)hipFree(b->d_queue);
     delete b;
@@ -807,7 +835,7 @@ tcv_status tcv_batch_grid(tcv_ctx *ctx, tcv_batch *b)
 tcv_status tcv_batch_degrid(tcv_ctx *ctx, tcv_batch *b)
 {
     if (!ctx || !b) return TCV_ERR_INVALID;
-    for (const JobDesc &d : b->host)
+    for (const JobDesc &d : b->hostD)
         if (!d.out) return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_batch_degrid: a job has no output");
     return launch_batch(ctx, b, true);
 }

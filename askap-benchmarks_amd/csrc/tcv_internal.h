@@ -113,6 +113,7 @@ bool grid2_supported(int tw, int th);
 struct PlanShape {
     int tw, th, chunk, k2, phases, xcd;
     int kern, dkern;       // gridder / degridder kernel (codes as in tcv_tuning)
+    int phasesG, xcdG;     // the gridder's work-item phases and XCD order
     int nTX, nTY, nTiles;
     int maxS;
     long long maxEntries;  // upper bound on visibility x tile pairs
@@ -134,6 +135,11 @@ struct tcv_plan {
     int *itemStart = nullptr;     // phases * nTiles + 1
     int *meta = nullptr;          // 32 ints: [0] items [1] queue head [2] entries [3] bad,
                                   // [8..15] per-XCD heads, [16..24] per-XCD item ranges
+    // the gridder's work items (phasesG); the same buffers as above when
+    // phasesG == phases
+    int4 *itemsG = nullptr;
+    long long maxItemsG = 0;
+    int *itemCountG = nullptr, *itemStartG = nullptr, *metaG = nullptr;
     int *scanTmp = nullptr;       // scratch for the multi-level scan
     long long scanTmpLen = 0;
     long long cSize = 0;
@@ -153,9 +159,9 @@ struct tcv_batch {
     int nJobs = 0;
     int tw = 0, th = 0, kern = kDefaultGridKernel, dkern = kDefaultDegridKernel;
     bool half = false;
-    std::vector<JobDesc> host;
+    std::vector<JobDesc> host, hostD;   // gridder / degridder descriptors
     std::vector<tcv_plan *> plans;
-    JobDesc *d_jobs = nullptr;
+    JobDesc *d_jobs = nullptr, *d_jobsD = nullptr;
     int *d_prefix = nullptr;      // nJobs + 1
     int *d_queue = nullptr;
     hipGraphExec_t graph = nullptr;
@@ -171,7 +177,7 @@ struct HalfStorage {
 
 tcv_status plan_build_kernels(tcv_ctx *ctx, tcv_plan *p, const GeomDev &g);
 // half == nullptr: FP32 taps and data
-JobDesc make_job_desc(const tcv_plan *p, const void *data, const void *C, float2 *grid,
+JobDesc make_job_desc(const tcv_plan *p, bool forGrid, const void *data, const void *C, float2 *grid,
                       float2 *out, const HalfStorage *half);
 tcv_status launch_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const void *data, const void *C,
                              float2 *grid, const HalfStorage *half);

@@ -1459,19 +1459,35 @@ tcv_status plan_build_kernels(tcv_ctx *ctx, tcv_plan *p, const GeomDev &g)
     TCV_HIP(ctx, hipMemcpyAsync(p->cursor, p->starts, sizeof(int) * sh.nKeys,
                                 hipMemcpyDeviceToDevice, ctx->stream));
     if ((st = bin(1)) != TCV_OK) return st;
-    const int slots = sh.phases * sh.nTiles;
-    hipLaunchKernelGGL(k_plan_items, dim3((slots + 255) / 256), dim3(256), 0, ctx->stream, 0, a,
-                       p->starts, p->itemCount, (const int *)p->itemStart, p->items);
-    TCV_HIP(ctx, hipGetLastError());
-    if ((st = scan_exclusive(ctx, p->itemCount, p->itemStart, slots, p->scanTmp)) != TCV_OK)
+    // work items: cuts of the sorted entries into (phase, tile) ranges
+    auto items = [&](int phases, int xcd, int *itemCount, int *itemStart, int4 *itemsOut,
+                     int *meta) -> tcv_status {
+        PlanDevArgs b = a;
+        b.phases = phases;
+        b.xcd = xcd;
+        const int slots = phases * sh.nTiles;
+        hipLaunchKernelGGL(k_plan_items, dim3((slots + 255) / 256), dim3(256), 0, ctx->stream, 0,
+                           b, p->starts, itemCount, (const int *)itemStart, itemsOut);
+        TCV_HIP(ctx, hipGetLastError());
+        tcv_status s2 = scan_exclusive(ctx, itemCount, itemStart, slots, p->scanTmp);
+        if (s2 != TCV_OK) return s2;
+        hipLaunchKernelGGL(k_plan_items, dim3((slots + 255) / 256), dim3(256), 0, ctx->stream, 1,
+                           b, p->starts, itemCount, (const int *)itemStart, itemsOut);
+        TCV_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(k_plan_meta, dim3(1), dim3(64), 0, ctx->stream, (const int *)itemStart,
+                           slots, (const int *)p->starts, sh.nKeys,
+                           xcd ? (phases / 8) * sh.nTiles : 0, meta);
+        TCV_HIP(ctx, hipGetLastError());
+        return TCV_OK;
+    };
+    if ((st = items(sh.phases, sh.xcd, p->itemCount, p->itemStart, p->items, p->meta)) != TCV_OK)
         return st;
-    hipLaunchKernelGGL(k_plan_items, dim3((slots + 255) / 256), dim3(256), 0, ctx->stream, 1, a,
-                       p->starts, p->itemCount, (const int *)p->itemStart, p->items);
-    TCV_HIP(ctx, hipGetLastError());
-    hipLaunchKernelGGL(k_plan_meta, dim3(1), dim3(64), 0, ctx->stream, (const int *)p->itemStart,
-                       slots, (const int *)p->starts, sh.nKeys,
-                       sh.xcd ? (sh.phases / 8) * sh.nTiles : 0, p->meta);
-    TCV_HIP(ctx, hipGetLastError());
+    if (p->metaG != p->meta) {
+        // the out-of-range count lives in meta[3]; the gridder's meta gets its own copy
+        if ((st = launch_fill_u32(ctx, p->metaG, 0, 32)) != TCV_OK) return st;
+        if ((st = items(sh.phasesG, sh.xcdG, p->itemCountG, p->itemStartG, p->itemsG, p->metaG)) != TCV_OK)
+            return st;
+    }
     return TCV_OK;
 }
 
@@ -1564,13 +1580,13 @@ bool tiled_config_supported(int tw, int th)
     return false;
 }
 
-JobDesc make_job_desc(const tcv_plan *p, const void *data, const void *C, float2 *grid,
+JobDesc make_job_desc(const tcv_plan *p, bool forGrid, const void *data, const void *C, float2 *grid,
                       float2 *out, const HalfStorage *half)
 {
     JobDesc d{};
-    d.items = p->items;
+    d.items = forGrid ? p->itemsG : p->items;
     d.entries = p->entries;
-    d.meta = p->meta;
+    d.meta = forGrid ? p->metaG : p->meta;
     d.data = data;
     d.C = C;
     d.grid = grid;
@@ -1588,9 +1604,9 @@ JobDesc make_job_desc(const tcv_plan *p, const void *data, const void *C, float2
 tcv_status launch_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const void *data, const void *C,
                              float2 *grid, const HalfStorage *half)
 {
-    ItemSource<false> src{make_job_desc(p, data, C, grid, nullptr, half), nullptr, nullptr, 0,
-                          p->meta + 1, p->sh.xcd};
-    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, p->meta);
+    ItemSource<false> src{make_job_desc(p, true, data, C, grid, nullptr, half), nullptr, nullptr, 0,
+                          p->metaG + 1, p->sh.xcdG};
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, p->metaG);
     TCV_HIP(ctx, hipGetLastError());
 #define X(a, b)                                                                        \
     if (p->sh.tw == a && p->sh.th == b)                                                \
@@ -1604,7 +1620,7 @@ tcv_status launch_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const void *data, const 
 tcv_status launch_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *grid, const void *C,
                                float2 *out, const HalfStorage *half)
 {
-    ItemSource<false> src{make_job_desc(p, nullptr, C, const_cast<float2 *>(grid), out, half),
+    ItemSource<false> src{make_job_desc(p, false, nullptr, C, const_cast<float2 *>(grid), out, half),
                           nullptr, nullptr, 0, p->meta + 1, p->sh.xcd};
     tcv_status st = launch_fill_u32(ctx, out, 0, (size_t)p->nVis * 2);
     if (st != TCV_OK) return st;
@@ -1622,14 +1638,16 @@ tcv_status launch_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *grid, co
 tcv_status launch_batch(tcv_ctx *ctx, tcv_batch *b, bool degrid)
 {
     if (degrid)
-        for (const JobDesc &d : b->host) {
+        for (const JobDesc &d : b->hostD) {
             tcv_status st = launch_fill_u32(ctx, d.out, 0, (size_t)d.nVis * 2);
             if (st != TCV_OK) return st;
         }
     hipLaunchKernelGGL(k_batch_prefix, dim3(1), dim3(64), 0, ctx->stream,
-                       (const JobDesc *)b->d_jobs, b->nJobs, b->d_prefix, b->d_queue);
+                       (const JobDesc *)(degrid ? b->d_jobsD : b->d_jobs), b->nJobs, b->d_prefix,
+                       b->d_queue);
     TCV_HIP(ctx, hipGetLastError());
-    ItemSource<true> src{JobDesc{}, b->d_jobs, b->d_prefix, b->nJobs, b->d_queue, 0};
+    ItemSource<true> src{JobDesc{}, degrid ? b->d_jobsD : b->d_jobs, b->d_prefix, b->nJobs,
+                         b->d_queue, 0};
     // batches run FP32 storage
 #define X(a, c)                                                                        \
     if (b->tw == a && b->th == c)                                                      \

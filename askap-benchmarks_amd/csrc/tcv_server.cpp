@@ -125,16 +125,19 @@ tcv_status launch_slot(tcv_server *s, Slot &sl, std::vector<Req *> &batch)
         if (r->r.op == 0) {
             st = launch_fill_u32(c, res, 0, (size_t)G.gSize * G.gSize * 2);
             if (st != TCV_OK) return st;
-            g.push_back(make_job_desc(&p, r->r.d_data, r->r.d_C, res, nullptr, nullptr));
+            g.push_back(make_job_desc(&p, true, r->r.d_data, r->r.d_C, res, nullptr, nullptr));
         } else {
-            d.push_back(make_job_desc(&p, nullptr, r->r.d_C, (float2 *)r->r.d_grid, res, nullptr));
+            d.push_back(make_job_desc(&p, false, nullptr, r->r.d_C, (float2 *)r->r.d_grid, res, nullptr));
         }
     }
     size_t k = 0;
     auto run = [&](tcv_batch &b, std::vector<JobDesc> &v, bool degrid) -> tcv_status {
         if (v.empty()) return TCV_OK;
         b.nJobs = (int)v.size();
-        b.host = v;
+        // one descriptor array per slot batch: the grid batch reads it as the
+        // gridder's, the degrid batch as the degridder's
+        (degrid ? b.hostD : b.host) = v;
+        b.d_jobsD = b.d_jobs;
         std::memcpy(sl.pinned + k, v.data(), sizeof(JobDesc) * v.size());
         TCV_HIP(c, hipMemcpyAsync(b.d_jobs, sl.pinned + k, sizeof(JobDesc) * v.size(),
                                   hipMemcpyHostToDevice, c->stream));

@@ -44,7 +44,7 @@ def alg_bytes(points, nVis, gSize, op):
 
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_grid_degrid_tiled_spectral.md")
 TUNING_KEYS = ("tile_w", "tile_h", "chunk", "slice_keys", "phases", "xcd_split", "kernel",
-               "degrid_kernel")
+               "degrid_kernel", "grid_phases")
 
 
 def measured_traffic(kernel, info):
@@ -264,6 +264,8 @@ def main():
     ap.add_argument("--xcd-split", type=int, default=-1)
     ap.add_argument("--kernel", type=int, default=0,
                     help="gridder: 1 register tile, 2 LDS tile, 3 register tile with tap pairs; 0 auto")
+    ap.add_argument("--grid-phases", type=int, default=0,
+                    help="C-slice phases of the gridder's work items (0 auto)")
     ap.add_argument("--degrid-kernel", type=int, default=0,
                     help="degridder: 1 register tile, 2 LDS tile, 3 LDS tile with tap pairs; 0 auto")
     ap.add_argument("--no-plain", action="store_true")
@@ -293,7 +295,7 @@ def main():
     ctx = tcv.Context(0)
     tuning = dict(tile_w=args.tile_w, tile_h=args.tile_h, chunk=args.chunk,
                   slice_keys=args.slice_keys, phases=args.phases, xcd_split=args.xcd_split,
-                  kernel=args.kernel, degrid_kernel=args.degrid_kernel)
+                  kernel=args.kernel, degrid_kernel=args.degrid_kernel, grid_phases=args.grid_phases)
     job = tcv.GridJob(ctx, prob, tuning=tuning)
     assert job.nBad == 0, f"{job.nBad} visibilities out of range"
     points = job.points

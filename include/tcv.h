@@ -196,6 +196,8 @@ typedef struct {
                         two columns per lane (16-byte C loads); 0 = auto */
     int degrid_kernel; /* degridder: 1 tile in registers; 2 tile in LDS, dense lanes;
                         3 LDS with tap pairs (16-byte C loads); 0 = auto */
+    int grid_phases; /* C-slice phases of the gridder's work items (the degridder uses
+                        phases; both cut the same sorted entries); 0 = auto */
 } tcv_tuning;
 
 typedef struct tcv_plan tcv_plan;

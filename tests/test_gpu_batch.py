@@ -99,8 +99,10 @@ def test_device_initC_matches_host_taps(ctx):
 
 @pytest.mark.parametrize("kernels", [(2, 2), (3, 3), (1, 3)])
 def test_batch_other_kernels_vs_oracle(ctx, kernels):
+    # gridder and degridder items cut at different phase counts
     bj = BatchJobs(ctx, small_specs(4, seed=21), tuning=dict(kernel=kernels[0],
-                                                              degrid_kernel=kernels[1]))
+                                                              degrid_kernel=kernels[1],
+                                                              phases=8, grid_phases=2))
     bj.zero_grids()
     bj.run_batched(graph=True)
     for j in range(len(bj.jobs)):

@@ -149,7 +149,9 @@ def test_tiled_shapes_vs_oracle(ctx, tw, th):
                         kernel=2, degrid_kernel=1),
                    dict(tile_w=tw, tile_h=th, kernel=3, degrid_kernel=3),
                    dict(tile_w=tw, tile_h=th, chunk=96, slice_keys=320, phases=16, xcd_split=1,
-                        kernel=3, degrid_kernel=3)):
+                        kernel=3, degrid_kernel=3),
+                   dict(tile_w=tw, tile_h=th, chunk=96, slice_keys=320, phases=16, grid_phases=4,
+                        xcd_split=1, kernel=3, degrid_kernel=3)):
         g = gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning)
         assert rel_err(g, gref) <= TOL, tuning
         out = gpu_degrid(ctx, vis, gref, C, "tiled", tuning)
@@ -340,7 +342,8 @@ def test_lds_and_register_kernels_bitwise_on_exact_inputs(ctx, tw, th):
     oref = O.degrid(G, iu, iv, co, C, Sz)
     for kernel in (1, 2, 3):
         for t in (dict(chunk=1 << 20, slice_keys=1, phases=1),
-                  dict(chunk=64, slice_keys=64, phases=8, xcd_split=1)):
+                  dict(chunk=64, slice_keys=64, phases=8, xcd_split=1),
+                  dict(chunk=64, slice_keys=64, phases=16, grid_phases=3, xcd_split=1)):
             tuning = dict(t, tile_w=tw, tile_h=th, kernel=kernel, degrid_kernel=kernel)
             assert np.array_equal(gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning), gref), tuning
             assert np.array_equal(gpu_degrid(ctx, vis, G, C, "tiled", tuning), oref), tuning
@@ -379,8 +382,9 @@ def test_tuning_kernel_field_validated(ctx):
         tcv.Plan(ctx, vis, kernel=4)
     with pytest.raises(tcv.TcvError):
         tcv.Plan(ctx, vis, degrid_kernel=7)
-    info = tcv.Plan(ctx, vis, kernel=2, degrid_kernel=3).info()
-    assert (info["kernel"], info["degrid_kernel"]) == (2, 3)
+    info = tcv.Plan(ctx, vis, kernel=2, degrid_kernel=3, slice_keys=64, phases=16,
+                    grid_phases=4).info()
+    assert (info["kernel"], info["degrid_kernel"], info["phases"], info["grid_phases"]) == (2, 3, 16, 4)
     # the two-column gridder needs 4 waves x 128/tile_w rows: (8, 32) falls back
     assert tcv.Plan(ctx, vis, tile_w=8, tile_h=32, kernel=3).info()["kernel"] == 1
 
