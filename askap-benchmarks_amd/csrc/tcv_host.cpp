@@ -95,6 +95,8 @@ tcv_status tcv_create(int device, tcv_ctx **out)
         return fail(TCV_ERR_DEVICE);
     if (hipEventCreate(&ctx->timer0) != hipSuccess || hipEventCreate(&ctx->timer1) != hipSuccess)
         return fail(TCV_ERR_DEVICE);
+    if (hipMalloc(&ctx->d_bcast, kMaxPersistentBlocks * sizeof(int)) != hipSuccess)
+        return fail(TCV_ERR_NOMEM);
     if (hipMalloc(&ctx->d_counters, 8 * sizeof(unsigned long long)) != hipSuccess)
         return fail(TCV_ERR_NOMEM);
     if (hipMemset(ctx->d_counters, 0, 8 * sizeof(unsigned long long)) != hipSuccess)
@@ -116,6 +118,7 @@ void tcv_destroy(tcv_ctx *ctx)
     if (ctx->timer0) (void)hipEventDestroy(ctx->timer0);
     if (ctx->timer1) (void)hipEventDestroy(ctx->timer1);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+    if (ctx->d_bcast) (void)hipFree(ctx->d_bcast);
     major_release(ctx);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -23,6 +23,8 @@ struct tcv_ctx {
     std::map<std::string, Stat> stats;
     // device counters: [0] visibilities skipped by the plain kernels (range)
     unsigned long long *d_counters = nullptr;
+    // one int per persistent workgroup: work-item hand-over without LDS
+    int *d_bcast = nullptr;
     // autotuner results per workload class
     std::map<std::string, tcv_tuning> tuned;
     // major cycle: FFT plans per grid size, grow-only Hogbom workspace
@@ -102,6 +104,8 @@ struct JobDesc {
     float dataScale;            // FP16: visibility scale
     int nVis;
 };
+
+constexpr int kMaxPersistentBlocks = 8192;
 
 // Tiled kernels used when a tuning leaves the choice open.  Gridder: 1 tile in
 // registers, 2 tile in LDS, 3 registers with two columns per lane.  Degridder:

@@ -564,6 +564,7 @@ struct ItemSource {
     int *queue;
 
     int xcd;   // single job: items are XCD-major with per-XCD queues
+    int *bcast;   // one slot per workgroup (kernels that keep LDS for their tile)
 
     __device__ __forceinline__ int total() const
     {
@@ -602,6 +603,24 @@ struct ItemSource {
         }
     }
 };
+
+// Next work item for the whole workgroup without spending LDS: thread 0 takes
+// it from the queue and publishes it in the workgroup's slot (the fence
+// completes the store before the barrier); the waves read the slot from L2.
+// The caller reaches another barrier before thread 0 publishes again, so no
+// wave can miss a value.
+template <bool BATCH>
+__device__ __forceinline__ int acquire_for_block(const ItemSource<BATCH> &src)
+{
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(src.bcast + blockIdx.x, src.acquire(), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+    }
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(src.bcast + blockIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
 
 // ---- tiled gridder ------------------------------------------------------------
 // attic/tConvolveMPI/Benchmark.cc:131-154 reorganised as owner-computes: a
@@ -961,16 +980,13 @@ __global__ __launch_bounds__(256) void k_grid_lds(ItemSource<BATCH> src)
     constexpr unsigned TB = Taps<HALF>::kBytes;
     constexpr int OWN = (TH / 4) * TW;   // cells owned by one wave
     __shared__ float2 tile[TH * TW];
-    __shared__ int s_item;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nItems = src.total();
 
     for (;;) {
-        if (threadIdx.x == 0) s_item = src.acquire();
-        __syncthreads();
-        const int item = __builtin_amdgcn_readfirstlane(s_item);
-        __syncthreads();
+        const int item = acquire_for_block(src);
+        __syncthreads();   // every wave has read the slot
         if (item >= nItems) break;
         JobDesc jd;
         int local;
@@ -1053,15 +1069,14 @@ __global__ __launch_bounds__(256) void k_degrid_lds(ItemSource<BATCH> src)
 {
     constexpr unsigned TB = Taps<HALF>::kBytes;
     __shared__ float2 tile[TH * TW];
-    __shared__ int s_item;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nItems = src.total();
 
     for (;;) {
-        if (threadIdx.x == 0) s_item = src.acquire();
-        __syncthreads();   // also: every wave is done with the previous tile
-        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        // the barrier inside also means every wave is done with the previous
+        // tile; the one after the tile load, that every wave has read the slot
+        const int item = acquire_for_block(src);
         if (item >= nItems) break;
         JobDesc jd;
         int local;
@@ -1134,15 +1149,14 @@ __global__ __launch_bounds__(256) void k_degrid_lds2(ItemSource<BATCH> src)
 {
     constexpr unsigned TB = Taps<HALF>::kBytes;
     __shared__ float2 tile[TH * TW];
-    __shared__ int s_item;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nItems = src.total();
 
     for (;;) {
-        if (threadIdx.x == 0) s_item = src.acquire();
-        __syncthreads();
-        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        // the barrier inside also means every wave is done with the previous
+        // tile; the one after the tile load, that every wave has read the slot
+        const int item = acquire_for_block(src);
         if (item >= nItems) break;
         JobDesc jd;
         int local;
@@ -1514,7 +1528,8 @@ int persistent_blocks(tcv_ctx *ctx, const void *kernel, int bs)
     int per = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, bs, 0) != hipSuccess || per < 1)
         per = 1;
-    return std::max(1, ctx->cus) * per;
+    // every block has a hand-over slot in the context
+    return std::min(std::max(1, ctx->cus) * per, kMaxPersistentBlocks);
 }
 
 __global__ void k_batch_prefix(const JobDesc *__restrict__ jobs, int nJobs, int *__restrict__ prefix,
@@ -1605,7 +1620,7 @@ tcv_status launch_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const void *data, const 
                              float2 *grid, const HalfStorage *half)
 {
     ItemSource<false> src{make_job_desc(p, true, data, C, grid, nullptr, half), nullptr, nullptr, 0,
-                          p->metaG + 1, p->sh.xcdG};
+                          p->metaG + 1, p->sh.xcdG, ctx->d_bcast};
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, p->metaG);
     TCV_HIP(ctx, hipGetLastError());
 #define X(a, b)                                                                        \
@@ -1621,7 +1636,7 @@ tcv_status launch_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *grid, co
                                float2 *out, const HalfStorage *half)
 {
     ItemSource<false> src{make_job_desc(p, false, nullptr, C, const_cast<float2 *>(grid), out, half),
-                          nullptr, nullptr, 0, p->meta + 1, p->sh.xcd};
+                          nullptr, nullptr, 0, p->meta + 1, p->sh.xcd, ctx->d_bcast};
     tcv_status st = launch_fill_u32(ctx, out, 0, (size_t)p->nVis * 2);
     if (st != TCV_OK) return st;
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, p->meta);
@@ -1647,7 +1662,7 @@ tcv_status launch_batch(tcv_ctx *ctx, tcv_batch *b, bool degrid)
                        b->d_queue);
     TCV_HIP(ctx, hipGetLastError());
     ItemSource<true> src{JobDesc{}, degrid ? b->d_jobsD : b->d_jobs, b->d_prefix, b->nJobs,
-                         b->d_queue, 0};
+                         b->d_queue, 0, ctx->d_bcast};
     // batches run FP32 storage
 #define X(a, c)                                                                        \
     if (b->tw == a && b->th == c)                                                      \
