@@ -915,32 +915,13 @@ std::string tuning_class(const tcv_geometry *g)
            std::to_string(lv) + "/" + std::to_string(g->d_wPlane ? 1 : 0);
 }
 
-static std::vector<tcv_tuning> builtin_candidates(const tcv_geometry *g)
+// Tile shapes worth trying for the widest kernel of a job.
+static std::vector<std::pair<int, int>> candidate_tiles(int maxS)
 {
-    int maxS = 0, minS = 1 << 30;
-    for (int k = 0; k < g->nPlanes; k++) {
-        maxS = std::max(maxS, g->sSize[k]);
-        minS = std::min(minS, g->sSize[k]);
-    }
-    const long long slices = std::max<long long>(1, g->cSize / ((long long)minS * minS));
-    const int k2 = (int)std::min<long long>(8192, slices);
-    const int ph = default_phases(g->cSize, k2, g, maxS, 64, 64);
-    std::vector<std::pair<int, int>> tiles;
-    if (maxS >= 48) tiles = {{64, 64}, {64, 32}};
-    else if (maxS >= 24) tiles = {{32, 64}, {32, 32}, {64, 32}};
-    else if (maxS >= 12) tiles = {{16, 64}, {16, 32}, {32, 32}};
-    else tiles = {{8, 64}, {8, 32}, {16, 32}};
-    std::vector<tcv_tuning> c;
-    for (auto &t : tiles)
-        for (int p : {ph / 2, ph, ph * 2}) {
-            if (p < 1 || p > k2) continue;
-            for (int kern : {1, 3})
-                for (int dk : {2, 3})
-                    c.push_back(tcv_tuning{t.first, t.second, 2048, k2, p, p >= 8 ? 1 : 0, kern, dk});
-        }
-    c.push_back(tcv_tuning{tiles[0].first, tiles[0].second, 512, k2, ph, ph >= 8 ? 1 : 0, 0, 0});
-    c.push_back(tcv_tuning{tiles[0].first, tiles[0].second, 2048, 1, 1, 0, 0, 0});   // no ordering
-    return c;
+    if (maxS >= 48) return {{64, 64}, {64, 32}, {32, 64}};
+    if (maxS >= 24) return {{32, 64}, {32, 32}, {64, 32}};
+    if (maxS >= 12) return {{16, 64}, {16, 32}, {32, 32}};
+    return {{8, 64}, {8, 32}, {16, 32}};
 }
 
 extern "C" {
@@ -955,31 +936,76 @@ tcv_status tcv_autotune(tcv_ctx *ctx, const tcv_geometry *g, const float *d_data
     if (op < 1 || op > 3 || reps < 1 || !best || !d_C || ((op & 1) && (!d_data || !d_grid)) ||
         ((op & 2) && (!d_grid || !d_out)))
         return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_autotune: bad argument");
-    std::vector<tcv_tuning> cand = candidates ? std::vector<tcv_tuning>(candidates, candidates + nCandidates)
-                                              : builtin_candidates(g);
     double bestT = 1e300;
     bool any = false;
-    for (const tcv_tuning &t : cand) {
+    // times one tuning on the job; false when it cannot be planned
+    auto measure = [&](const tcv_tuning &t, double *msOut) -> tcv_status {
+        *msOut = -1.0;
         tcv_plan *p = nullptr;
-        if (tcv_plan_create(ctx, g, &t, &p) != TCV_OK) continue;   // skip shapes that do not fit
+        if (tcv_plan_create(ctx, g, &t, &p) != TCV_OK) return TCV_OK;   // shape does not fit
         auto once = [&]() -> tcv_status {
             tcv_status s2 = TCV_OK;
             if (op & 1) s2 = tcv_grid_tiled(ctx, p, d_data, d_C, d_grid);
             if (s2 == TCV_OK && (op & 2)) s2 = tcv_degrid_tiled(ctx, p, d_grid, d_C, d_out);
             return s2;
         };
-        st = once();
+        tcv_status s3 = once();
         double ms = 0;
-        if (st == TCV_OK) st = tcv_timer_start(ctx);
-        for (int r = 0; r < reps && st == TCV_OK; r++) st = once();
-        if (st == TCV_OK) st = tcv_timer_stop(ctx, &ms);
+        if (s3 == TCV_OK) s3 = tcv_timer_start(ctx);
+        for (int r = 0; r < reps && s3 == TCV_OK; r++) s3 = once();
+        if (s3 == TCV_OK) s3 = tcv_timer_stop(ctx, &ms);
         tcv_plan_destroy(ctx, p);
-        if (st != TCV_OK) return st;
-        ms /= reps;
-        if (ms < bestT) {
+        if (s3 == TCV_OK) *msOut = ms / reps;
+        return s3;
+    };
+    auto consider = [&](const tcv_tuning &t) -> tcv_status {
+        double ms;
+        tcv_status s4 = measure(t, &ms);
+        if (s4 == TCV_OK && ms >= 0 && ms < bestT) {
             bestT = ms;
             *best = t;
             any = true;
+        }
+        return s4;
+    };
+    if (candidates) {
+        for (int k = 0; k < nCandidates; k++)
+            if ((st = consider(candidates[k])) != TCV_OK) return st;
+    } else {
+        // staged search from the defaults: tile shape x kernel family, then the
+        // phase counts of the degridder and gridder items, then the item size
+        tcv_tuning d{0, 0, 0, 0, 0, -1, 0, 0, 0};
+        auto_tuning(maxS, g, &d);
+        for (auto &ts : candidate_tiles(maxS))
+            for (auto kk : {std::make_pair(3, 3), std::make_pair(1, 2), std::make_pair(2, 3)}) {
+                tcv_tuning t = d;
+                t.tile_w = ts.first;
+                t.tile_h = ts.second;
+                t.kernel = kk.first;
+                t.degrid_kernel = kk.second;
+                t.phases = 0;
+                t.grid_phases = 0;
+                t.xcd_split = -1;
+                auto_tuning(maxS, g, &t);   // phases for this tile shape
+                if ((st = consider(t)) != TCV_OK) return st;
+            }
+        if (!any) return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tcv_autotune: no candidate could be planned");
+        const tcv_tuning b1 = *best;
+        for (int p : {b1.phases / 2, b1.phases, b1.phases * 2})
+            for (int gp : {p, p / 2}) {
+                if (p < 1 || gp < 1 || p > b1.slice_keys || (p == b1.phases && gp == b1.grid_phases))
+                    continue;
+                tcv_tuning t = b1;
+                t.phases = p;
+                t.grid_phases = gp;
+                t.xcd_split = p >= 8 ? 1 : 0;
+                if ((st = consider(t)) != TCV_OK) return st;
+            }
+        const tcv_tuning b2 = *best;
+        for (int ch : {512, 8192}) {
+            tcv_tuning t = b2;
+            t.chunk = ch;
+            if ((st = consider(t)) != TCV_OK) return st;
         }
     }
     if (!any) return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tcv_autotune: no candidate could be planned");

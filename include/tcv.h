@@ -222,11 +222,11 @@ tcv_status tcv_plan_info(tcv_ctx *ctx, tcv_plan *plan, tcv_tuning *tuning,
 /* Autotuner: times candidate tunings of the tiled path on this job (hipEvents
  * on the context stream, `reps` launches each after one warm-up) and returns
  * the fastest in *best (and its average milliseconds in *bestMs).  op: 1 =
- * gridding, 2 = degridding, 3 = both (summed).  candidates == NULL uses the
- * built-in set: tile shapes suited to the kernel width, C-slice phase counts
- * around the L2-sized target (about 4 MB of C per phase) with and without the
- * XCD split, and two work-item sizes.  d_grid and d_out are scratch (their
- * contents are overwritten).  The winner is cached in the context for the
+ * gridding, 2 = degridding, 3 = both (summed).  candidates == NULL runs a
+ * staged search from the defaults: tile shapes suited to the kernel width x
+ * the kernel families, then the degridder's and gridder's C-slice phase
+ * counts around the L2-sized target, then the work-item size.  d_grid and
+ * d_out are scratch (their contents are overwritten).  The winner is cached in the context for the
  * workload class (grid size, widest kernel, table size, visibility count in
  * powers of two) and used by tcv_plan_create whenever tuning is NULL. Blocking. */
 tcv_status tcv_autotune(tcv_ctx *ctx, const tcv_geometry *g, const float *d_data,
