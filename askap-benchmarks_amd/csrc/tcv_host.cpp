@@ -490,6 +490,7 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
         if (sh.kern == 3) sh.kern = 1;
     }
     if (minS < 2 && sh.dkern == 3) sh.dkern = 2;
+    sh.minS = minS;
     sh.nTX = (g->gSize + sh.tw - 1) / sh.tw;
     sh.nTY = (g->gSize + sh.th - 1) / sh.th;
     sh.nTiles = sh.nTX * sh.nTY;

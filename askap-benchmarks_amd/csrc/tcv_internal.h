@@ -118,6 +118,7 @@ struct PlanShape {
     int tw, th, chunk, k2, phases, xcd;
     int kern, dkern;       // gridder / degridder kernel (codes as in tcv_tuning)
     int phasesG, xcdG;     // the gridder's work-item phases and XCD order
+    int minS;              // narrowest kernel width of the job
     int nTX, nTY, nTiles;
     int maxS;
     long long maxEntries;  // upper bound on visibility x tile pairs

@@ -533,6 +533,41 @@ __device__ __forceinline__ float4 load_pair(__amdgpu_buffer_rsrc_t r, unsigned v
     }
 }
 
+// Four consecutive FP16 taps in one 16-byte load.
+struct Quad {
+    float2 t0, t1, t2, t3;
+};
+
+__device__ __forceinline__ float2 half_tap(unsigned w)
+{
+    const half2_t h = __builtin_bit_cast(half2_t, w);
+    return make_float2((float)h.x, (float)h.y);
+}
+
+__device__ __forceinline__ Quad load_quad_half(__amdgpu_buffer_rsrc_t r, unsigned voff)
+{
+    const uint4 u = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0));
+    return Quad{half_tap(u.x), half_tap(u.y), half_tap(u.z), half_tap(u.w)};
+}
+
+// a or b by a lane-dependent flag, as a bit blend: a plain select of two
+// struct members can be rewritten by the compiler into an indexed load from
+// the stack, which would put the quadruples in scratch memory
+__device__ __forceinline__ float blend(bool pickB, float a, float b)
+{
+    const unsigned m = 0u - (unsigned)pickB;
+    return __uint_as_float((__float_as_uint(a) & ~m) | (__float_as_uint(b) & m));
+}
+
+// tap i of a quadruple for a lane-dependent i in [0, 4)
+__device__ __forceinline__ float2 pick4(const Quad &q, int i)
+{
+    const bool lo = i & 1, hi = i & 2;
+    const float ax = blend(lo, q.t0.x, q.t1.x), ay = blend(lo, q.t0.y, q.t1.y);
+    const float bx = blend(lo, q.t2.x, q.t3.x), by = blend(lo, q.t2.y, q.t3.y);
+    return make_float2(blend(hi, ax, bx), blend(hi, ay, by));
+}
+
 __device__ __forceinline__ int entry_width(int w) { return w & 0xFFFF; }
 __device__ __forceinline__ int entry_plane(const int4 &en, const JobDesc &d)
 {
@@ -952,6 +987,124 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled2(ItemSource<BATCH> src)
     }
 }
 
+// ---- FP16 storage: four taps per lane ------------------------------------
+// With half (re, im) taps a 16-byte load carries four of them, so in FP16
+// storage mode the register gridder gives each lane four adjacent tile columns
+// and the LDS degridder walks footprints four columns at a time.  Edge lanes
+// load the quadruple that starts (or ends) on the footprint's edge tap, so
+// every load stays inside the visibility's C slice (kernel widths >= 4).
+template <int TW, int R, int NW, int RB, bool BATCH>
+__global__ __launch_bounds__(NW * 64) void k_grid_tiled4h(ItemSource<BATCH> src)
+{
+    constexpr int V = 4;
+    constexpr int LPR = TW / V;      // lanes per tile row
+    constexpr int RPI = 64 / LPR;    // tile rows per instruction
+    constexpr int TH = NW * R * RPI;
+    constexpr unsigned TB = 4;
+    static_assert(R % RB == 0, "row block must divide the register rows");
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lx = V * (lane % LPR), ly = lane / LPR;
+    __shared__ int s_item;
+    const int nItems = src.total();
+    const int q0 = wave * R;
+
+    for (;;) {
+        if (threadIdx.x == 0) s_item = src.acquire();
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (item >= nItems) break;
+        JobDesc jd;
+        int local;
+        src.locate(item, jd, local);
+        const __amdgpu_buffer_rsrc_t rc = make_rsrc(jd.C, jd.cBytes);
+        const int4 *__restrict__ entries = jd.entries;
+        const int gSize = jd.gSize;
+        const int4 it = jd.items[local];
+        const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
+        const int Bx = txi * TW, By = tyi * TH;
+        const int e0 = it.y, e1 = it.y + it.z;
+
+        float ar[R][V], ai[R][V];
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+#pragma unroll
+            for (int c = 0; c < V; ++c) ar[j][c] = ai[j][c] = 0.f;
+
+        int4 enA = entries[e0];
+        int4 enB = entries[min(e0 + 1, e1 - 1)];
+        float2 dA = vis_value<true>(jd, enA);
+        for (int e = e0; e < e1; ++e) {
+            const float2 dB = vis_value<true>(jd, enB);
+            const int4 enC = entries[min(e + 2, e1 - 1)];
+            const int u0 = enA.x & 0xFFFF, v0 = (int)((unsigned)enA.x >> 16);
+            const int c0 = enA.y, s = entry_width(enA.w);
+            const int cx = Bx + lx - u0;               // C column of the lane's first cell
+            const int start = min(max(cx, 0), s - V);  // the quadruple loaded
+            const int off = cx - start;                // tap of cell c is off + c
+            const unsigned lanev = (cx < s && cx + V > 0) ? (unsigned)(c0 + start) * TB : kOOB;
+            bool ok[V];
+#pragma unroll
+            for (int c = 0; c < V; ++c) ok[c] = (unsigned)(cx + c) < (unsigned)s;
+            const int dy = By - v0;
+#pragma unroll
+            for (int jb = 0; jb < R; jb += RB) {
+                const int r0 = (q0 + jb) * RPI;
+                if (r0 + RB * RPI + dy > 0 && r0 + dy < s) {
+                    Quad t[RB];
+#pragma unroll
+                    for (int k = 0; k < RB; ++k) {
+                        const int cy = r0 + k * RPI + ly + dy;
+                        const unsigned vo = ((unsigned)cy < (unsigned)s)
+                                                ? lanev + (unsigned)(cy * s) * TB : kOOB;
+                        t[k] = load_quad_half(rc, vo);
+                    }
+#pragma unroll
+                    for (int k = 0; k < RB; ++k)
+#pragma unroll
+                        for (int c = 0; c < V; ++c) {
+                            float2 tc = pick4(t[k], off + c);
+                            if (!ok[c]) tc = make_float2(0.f, 0.f);
+                            ar[jb + k][c] = fmaf(dA.x, tc.x, fmaf(-dA.y, tc.y, ar[jb + k][c]));
+                            ai[jb + k][c] = fmaf(dA.x, tc.y, fmaf(dA.y, tc.x, ai[jb + k][c]));
+                        }
+                }
+            }
+            enA = enB;
+            dA = dB;
+            enB = enC;
+        }
+
+        const int col = Bx + lx;
+        const int rowsLeft = gSize - (By + q0 * RPI + ly);
+        float2 *__restrict__ g0 = jd.grid + (size_t)(By + q0 * RPI + ly) * gSize + col;
+        if (it.w) {
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+#pragma unroll
+                for (int c = 0; c < V; ++c) {
+                    float2 *gp = g0 + (size_t)j * RPI * gSize + c;
+                    if (j * RPI < rowsLeft && col + c < gSize && (ar[j][c] != 0.f || ai[j][c] != 0.f)) {
+                        const float2 o = *gp;
+                        *gp = make_float2(o.x + ar[j][c], o.y + ai[j][c]);
+                    }
+                }
+        } else {
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+#pragma unroll
+                for (int c = 0; c < V; ++c) {
+                    float *fp = reinterpret_cast<float *>(g0 + (size_t)j * RPI * gSize + c);
+                    if (j * RPI < rowsLeft && col + c < gSize && (ar[j][c] != 0.f || ai[j][c] != 0.f)) {
+                        unsafeAtomicAdd(fp, ar[j][c]);
+                        unsafeAtomicAdd(fp + 1, ai[j][c]);
+                    }
+                }
+        }
+    }
+}
+
 // ---- tiled gridder / degridder, LDS tile with dense lanes --------------------
 // The same work items as above, with the tile in LDS so that lanes map to the
 // points of each visibility's footprint on the tile rather than to tile
@@ -1239,6 +1392,89 @@ __global__ __launch_bounds__(256) void k_degrid_lds2(ItemSource<BATCH> src)
     }
 }
 
+// FP16 degridder: LDS tile, footprints walked four columns at a time (one
+// 16-byte load of four half taps per lane and group).
+template <int TW, int TH, bool BATCH>
+__global__ __launch_bounds__(256) void k_degrid_lds4h(ItemSource<BATCH> src)
+{
+    constexpr int V = 4;
+    constexpr unsigned TB = 4;
+    __shared__ float2 tile[TH * TW];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nItems = src.total();
+
+    for (;;) {
+        // the barrier inside also means every wave is done with the previous
+        // tile; the one after the tile load, that every wave has read the slot
+        const int item = acquire_for_block(src);
+        if (item >= nItems) break;
+        JobDesc jd;
+        int local;
+        src.locate(item, jd, local);
+        const __amdgpu_buffer_rsrc_t rc = make_rsrc(jd.C, jd.cBytes);
+        const int4 *__restrict__ entries = jd.entries;
+        const float2 *__restrict__ grid = jd.grid;
+        const int gSize = jd.gSize;
+        const int4 it = jd.items[local];
+        const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
+        const int Bx = txi * TW, By = tyi * TH;
+        for (int q = threadIdx.x; q < TH * TW; q += 256) {
+            const int row = By + q / TW, col = Bx + q % TW;
+            tile[q] = (row < gSize && col < gSize) ? grid[(size_t)row * gSize + col]
+                                                   : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+
+        for (int e = it.y + wave; e < it.y + it.z; e += 4) {
+            const int4 en = entries[e];
+            const int u0 = en.x & 0xFFFF, v0 = (int)((unsigned)en.x >> 16);
+            const int c0 = en.y, s = entry_width(en.w);
+            const int colLo = max(u0 - Bx, 0), colHi = min(u0 + s - Bx, TW);
+            const int rowLo = max(v0 - By, 0), rowHi = min(v0 + s - By, TH);
+            const int w = colHi - colLo, h = rowHi - rowLo;
+            if (w <= 0 || h <= 0) continue;
+            const int wq = (w + V - 1) / V;   // groups per row
+            const int n = wq * h;
+            const float rw = 1.0f / (float)wq;
+            const int cxLo = Bx + colLo - u0;
+            const int cRow = c0 + (By + rowLo - v0) * s;
+            const float2 *tBase = tile + rowLo * TW + colLo;
+            float sr = 0.f, si = 0.f;
+            for (int p0 = lane; p0 < n; p0 += 64 * kLdsU) {
+#pragma unroll
+                for (int u = 0; u < kLdsU; ++u) {
+                    const int p = p0 + 64 * u;
+                    int rl, k;
+                    split_flat(p, rw, wq, rl, k);
+                    const int x = V * k;
+                    const int cx = cxLo + x;
+                    const int start = min(cx, s - V);   // s >= 4 for this kernel
+                    const int off = cx - start;
+                    const Quad t = load_quad_half(rc, p < n ? (unsigned)(cRow + rl * s + start) * TB : kOOB);
+                    const float2 *tp = tBase + rl * TW + x;
+#pragma unroll
+                    for (int c = 0; c < V; ++c) {
+                        const bool ok = p < n && x + c < w;
+                        const float2 g = ok ? tp[c] : make_float2(0.f, 0.f);
+                        const float2 tc = pick4(t, off + c);
+                        sr = fmaf(g.x, tc.x, fmaf(-g.y, tc.y, sr));
+                        si = fmaf(g.x, tc.y, fmaf(g.y, tc.x, si));
+                    }
+                }
+            }
+            sr = wave_sum(sr);
+            si = wave_sum(si);
+            const float ps = jd.planeScale[entry_plane(en, jd)];
+            if (lane == 0) {
+                float *op = reinterpret_cast<float *>(jd.out + en.z);
+                unsafeAtomicAdd(op, sr * ps);
+                unsafeAtomicAdd(op + 1, si * ps);
+            }
+        }
+    }
+}
+
 // ---- FP16 storage conversion ----------------------------------------------------
 // Per-plane max |component| of C, then taps / scale rounded to half.  Each thread
 // walks a contiguous run of taps and flushes its running max with one atomicMax
@@ -1521,6 +1757,7 @@ struct TileCfg {
     static constexpr int RG = TH / (kNW * RPI);   // gridder rows groups per wave
     static constexpr int RD = TH / RPI;           // degridder row groups per wave
     static constexpr int RG2 = TH / (kNW * (128 / TW));   // two-column gridder (0: not built)
+    static constexpr int RG4 = TH * TW / (kNW * 256);      // four-column FP16 gridder (0: not built)
 };
 
 int persistent_blocks(tcv_ctx *ctx, const void *kernel, int bs)
@@ -1552,7 +1789,13 @@ tcv_status grid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_kin
     using T = TileCfg<TW, TH>;
     auto kern = kern_kind == 2 ? k_grid_lds<TW, TH, HALF, BATCH>
                                : k_grid_tiled<TW, T::RG, kNW, (T::RG % kRB == 0 ? kRB : T::RG), HALF, BATCH>;
-    if constexpr (T::RG2 >= 1) {
+    if constexpr (HALF) {
+        // FP16 storage: four taps per 16-byte load where the shape allows it
+        if constexpr (T::RG4 >= 1) {
+            if (kern_kind == 3)
+                kern = k_grid_tiled4h<TW, T::RG4, kNW, (T::RG4 % kRB == 0 ? kRB : T::RG4), BATCH>;
+        }
+    } else if constexpr (T::RG2 >= 1) {
         if (kern_kind == 3)
             kern = k_grid_tiled2<TW, T::RG2, kNW, (T::RG2 % kRB == 0 ? kRB : T::RG2), HALF, BATCH>;
     }
@@ -1569,7 +1812,10 @@ tcv_status degrid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_k
     using T = TileCfg<TW, TH>;
     auto kern = kern_kind == 2 ? k_degrid_lds<TW, TH, HALF, BATCH>
                                : k_degrid_tiled<TW, T::RD, kNW, (T::RD % kRB == 0 ? kRB : T::RD), HALF, BATCH>;
-    if (kern_kind == 3) kern = k_degrid_lds2<TW, TH, HALF, BATCH>;
+    if (kern_kind == 3) {
+        if constexpr (HALF) kern = k_degrid_lds4h<TW, TH, BATCH>;
+        else kern = k_degrid_lds2<TW, TH, HALF, BATCH>;
+    }
     const int blocks = persistent_blocks(ctx, (const void *)kern, kNW * 64);
     ProfScope ps(ctx, BATCH ? "degrid_batch" : (HALF ? "degrid_tiled_half" : "degrid_tiled"));
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kNW * 64), 0, ctx->stream, src);
@@ -1623,10 +1869,12 @@ tcv_status launch_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const void *data, const 
                           p->metaG + 1, p->sh.xcdG, ctx->d_bcast};
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, p->metaG);
     TCV_HIP(ctx, hipGetLastError());
+    // the FP16 four-tap gridder needs kernel widths of at least 4
+    const int kern = (half && p->sh.kern == 3 && p->sh.minS < 4) ? 1 : p->sh.kern;
 #define X(a, b)                                                                        \
     if (p->sh.tw == a && p->sh.th == b)                                                \
-        return half ? grid_tiled_t<a, b, true, false>(ctx, src, p->sh.kern)            \
-                    : grid_tiled_t<a, b, false, false>(ctx, src, p->sh.kern);
+        return half ? grid_tiled_t<a, b, true, false>(ctx, src, kern)                  \
+                    : grid_tiled_t<a, b, false, false>(ctx, src, kern);
     TCV_TILE_CONFIGS(X)
 #undef X
     return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");
@@ -1641,10 +1889,12 @@ tcv_status launch_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *grid, co
     if (st != TCV_OK) return st;
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, p->meta);
     TCV_HIP(ctx, hipGetLastError());
+    // the FP16 four-tap degridder needs kernel widths of at least 4
+    const int kern = (half && p->sh.dkern == 3 && p->sh.minS < 4) ? 2 : p->sh.dkern;
 #define X(a, b)                                                                        \
     if (p->sh.tw == a && p->sh.th == b)                                                \
-        return half ? degrid_tiled_t<a, b, true, false>(ctx, src, p->sh.dkern)         \
-                    : degrid_tiled_t<a, b, false, false>(ctx, src, p->sh.dkern);
+        return half ? degrid_tiled_t<a, b, true, false>(ctx, src, kern)                \
+                    : degrid_tiled_t<a, b, false, false>(ctx, src, kern);
     TCV_TILE_CONFIGS(X)
 #undef X
     return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");

@@ -90,3 +90,28 @@ def test_half_config1_errors_reported(ctx):
     r16 = np.sqrt(np.mean(np.abs(g16 - g64) ** 2)) / m
     print(f"config1 grid vs fp64: fp32 max {e32:.2e}, fp16 max {e16:.2e} rms {r16:.2e}")
     assert e32 <= 1e-5 and e16 <= HALF_TOL
+
+
+@pytest.mark.parametrize("Sz", [3, 5, 33])
+@pytest.mark.parametrize("tw,th", [(64, 64), (32, 32), (16, 32), (8, 64)])
+def test_half_four_tap_kernels_shapes(ctx, Sz, tw, th):
+    """FP16 storage with the four-tap kernels at every tile shape (shapes too
+    narrow for four columns per lane, and kernels narrower than four taps,
+    fall back to the other kernels): within the FP16 tolerance of FP64."""
+    gSize, os_, planes = 192, 4, 3
+    iu, iv, co, data, C = small_problem(700 + Sz + tw, 1500, gSize, Sz, os_=os_, nPlanes=planes,
+                                        cluster=50)
+    vis = tcv.Visibilities(ctx, iu, iv, co, [Sz], gSize, C.size)
+    d_C, d_data = ctx.to_device(C), ctx.to_device(data)
+    Ch = tcv.HalfTable(ctx, d_C, planes, Sz * Sz * os_ * os_)
+    dh = tcv.HalfVis(ctx, d_data)
+    plan = tcv.Plan(ctx, vis, tile_w=tw, tile_h=th, kernel=3, degrid_kernel=3, phases=2,
+                    slice_keys=8, chunk=64)
+    g = ctx.zeros((gSize, gSize), np.complex64)
+    plan.grid_half(dh, Ch, g)
+    g64 = O.grid_f64(data, iu, iv, co, C, gSize, Sz)
+    assert rel_err(g.download(), g64) <= HALF_TOL
+    out = ctx.empty(vis.nVis, np.complex64)
+    plan.degrid_half(ctx.to_device(g64.astype(np.complex64)), Ch, out)
+    o64 = O.degrid_f64(g64, iu, iv, co, C, Sz)
+    assert rel_err(out.download(), o64) <= HALF_TOL
