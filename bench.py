@@ -269,6 +269,7 @@ def main():
     ap.add_argument("--degrid-kernel", type=int, default=0,
                     help="degridder: 1 register tile, 2 LDS tile, 3 LDS tile with tap pairs; 0 auto")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-half", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=64)
@@ -361,6 +362,37 @@ def main():
         "roofline": roofline,
         "kernels": per_kernel,
     }
+
+    if not args.no_half:
+        # FP16 storage of C and the visibilities (per-w-plane scales, FP32
+        # accumulation) on the same plan: half the bytes per grid point
+        Ch = tcv.HalfTable(ctx, job.C, prob.wSize, int(prob.sSize[0]) ** 2 * 64)
+        dh = tcv.HalfVis(ctx, job.data)
+        gh = ctx.zeros((prob.gSize, prob.gSize), np.complex64)
+        oh = ctx.empty(prob.nVis, np.complex64)
+        job.plan.grid_half(dh, Ch, gh)
+        job.plan.degrid_half(gh, Ch, oh)
+        reps = max(1, min(args.steps, 3))
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(reps):
+            gh.zero()
+            job.plan.grid_half(dh, Ch, gh)
+        t_gh = ctx.timer_stop() / reps
+        ctx.timer_start()
+        for _ in range(reps):
+            job.plan.degrid_half(gh, Ch, oh)
+        t_dh = ctx.timer_stop() / reps
+        # agreement with the FP32 result of the timed steps
+        g32 = job.grid_d.download()
+        g16 = gh.download()
+        err = float(np.abs(g16 - g32).max() / np.abs(g32).max())
+        result["half"] = {"value": 2 * points / ((t_gh + t_dh) * 1e-3), "grid_ms": t_gh,
+                          "degrid_ms": t_dh, "alg_bytes_per_point": 4,
+                          "grid_max_rel_diff_vs_fp32": err}
+        log(f"fp16 storage: grid {t_gh:.1f} ms, degrid {t_dh:.1f} ms, "
+            f"{result['half']['value'] / 1e9:.1f} G points/s, max |fp16 - fp32| / max {err:.1e}")
+        del Ch, dh, gh, oh
 
     if not args.no_plain:
         job.zero_grid()
