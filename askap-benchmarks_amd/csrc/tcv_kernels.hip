@@ -640,17 +640,18 @@ struct ItemSource {
 };
 
 // Next work item for the whole workgroup without spending LDS: thread 0 takes
-// it from the queue and publishes it in the workgroup's slot (the fence
-// completes the store before the barrier); the waves read the slot from L2.
-// The caller reaches another barrier before thread 0 publishes again, so no
-// wave can miss a value.
+// it from the queue and publishes it in the workgroup's slot; waiting for its
+// memory counters completes the store in the CU's L2 before the barrier (a
+// device-scope fence would also write back the L2), and the waves, all on the
+// same CU, read the slot from that L2.  The caller reaches another barrier
+// before thread 0 publishes again, so no wave can miss a value.
 template <bool BATCH>
 __device__ __forceinline__ int acquire_for_block(const ItemSource<BATCH> &src)
 {
     if (threadIdx.x == 0) {
         __hip_atomic_store(src.bcast + blockIdx.x, src.acquire(), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
+        __builtin_amdgcn_s_waitcnt(0);   // every counter of this wave drained
     }
     __syncthreads();
     return __builtin_amdgcn_readfirstlane(
