@@ -479,17 +479,17 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
     sh.xcd = (t.xcd_split > 0 && sh.phases % 8 == 0) ? 1 : 0;
     sh.phasesG = std::min(t.grid_phases, sh.k2);
     sh.xcdG = (t.xcd_split > 0 && sh.phasesG % 8 == 0) ? 1 : 0;
-    if (t.kernel < 1 || t.kernel > 3 || t.degrid_kernel < 1 || t.degrid_kernel > 3)
-        return tcv_fail(ctx, TCV_ERR_INVALID, "tuning.kernel and degrid_kernel must be 0..3");
+    if (t.kernel < 1 || t.kernel > 4 || t.degrid_kernel < 1 || t.degrid_kernel > 4)
+        return tcv_fail(ctx, TCV_ERR_INVALID, "tuning.kernel and degrid_kernel must be 0..4");
     sh.kern = t.kernel;
     sh.dkern = t.degrid_kernel;
     // the pair kernels need kernel widths of at least 2 (a pair inside each C row)
     int minS = maxS;
     for (int k = 0; k < g->nPlanes; k++) minS = std::min(minS, g->sSize[k]);
-    if (minS < 2 || (sh.kern == 3 && !grid2_supported(sh.tw, sh.th))) {
-        if (sh.kern == 3) sh.kern = 1;
+    if (minS < 2 || (sh.kern >= 3 && !grid2_supported(sh.tw, sh.th))) {
+        if (sh.kern >= 3) sh.kern = 1;
     }
-    if (minS < 2 && sh.dkern == 3) sh.dkern = 2;
+    if (minS < 2 && sh.dkern >= 3) sh.dkern = 2;
     sh.minS = minS;
     sh.nTX = (g->gSize + sh.tw - 1) / sh.tw;
     sh.nTY = (g->gSize + sh.th - 1) / sh.th;

@@ -1790,15 +1790,14 @@ tcv_status grid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_kin
     using T = TileCfg<TW, TH>;
     auto kern = kern_kind == 2 ? k_grid_lds<TW, TH, HALF, BATCH>
                                : k_grid_tiled<TW, T::RG, kNW, (T::RG % kRB == 0 ? kRB : T::RG), HALF, BATCH>;
-    if constexpr (HALF) {
-        // FP16 storage: four taps per 16-byte load where the shape allows it
-        if constexpr (T::RG4 >= 1) {
-            if (kern_kind == 3)
-                kern = k_grid_tiled4h<TW, T::RG4, kNW, (T::RG4 % kRB == 0 ? kRB : T::RG4), BATCH>;
-        }
-    } else if constexpr (T::RG2 >= 1) {
-        if (kern_kind == 3)
+    if constexpr (T::RG2 >= 1) {
+        if (kern_kind >= 3)
             kern = k_grid_tiled2<TW, T::RG2, kNW, (T::RG2 % kRB == 0 ? kRB : T::RG2), HALF, BATCH>;
+    }
+    if constexpr (HALF && T::RG4 >= 1) {
+        // FP16 storage: four taps per 16-byte load
+        if (kern_kind == 4)
+            kern = k_grid_tiled4h<TW, T::RG4, kNW, (T::RG4 % kRB == 0 ? kRB : T::RG4), BATCH>;
     }
     const int blocks = persistent_blocks(ctx, (const void *)kern, kNW * 64);
     ProfScope ps(ctx, BATCH ? "grid_batch" : (HALF ? "grid_tiled_half" : "grid_tiled"));
@@ -1813,9 +1812,9 @@ tcv_status degrid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_k
     using T = TileCfg<TW, TH>;
     auto kern = kern_kind == 2 ? k_degrid_lds<TW, TH, HALF, BATCH>
                                : k_degrid_tiled<TW, T::RD, kNW, (T::RD % kRB == 0 ? kRB : T::RD), HALF, BATCH>;
-    if (kern_kind == 3) {
-        if constexpr (HALF) kern = k_degrid_lds4h<TW, TH, BATCH>;
-        else kern = k_degrid_lds2<TW, TH, HALF, BATCH>;
+    if (kern_kind >= 3) kern = k_degrid_lds2<TW, TH, HALF, BATCH>;
+    if constexpr (HALF) {
+        if (kern_kind == 4) kern = k_degrid_lds4h<TW, TH, BATCH>;   // four FP16 taps per load
     }
     const int blocks = persistent_blocks(ctx, (const void *)kern, kNW * 64);
     ProfScope ps(ctx, BATCH ? "degrid_batch" : (HALF ? "degrid_tiled_half" : "degrid_tiled"));
@@ -1871,7 +1870,7 @@ tcv_status launch_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const void *data, const 
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, p->metaG);
     TCV_HIP(ctx, hipGetLastError());
     // the FP16 four-tap gridder needs kernel widths of at least 4
-    const int kern = (half && p->sh.kern == 3 && p->sh.minS < 4) ? 1 : p->sh.kern;
+    const int kern = (half && p->sh.kern == 4 && p->sh.minS < 4) ? 3 : p->sh.kern;
 #define X(a, b)                                                                        \
     if (p->sh.tw == a && p->sh.th == b)                                                \
         return half ? grid_tiled_t<a, b, true, false>(ctx, src, kern)                  \
@@ -1891,7 +1890,7 @@ tcv_status launch_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *grid, co
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, p->meta);
     TCV_HIP(ctx, hipGetLastError());
     // the FP16 four-tap degridder needs kernel widths of at least 4
-    const int kern = (half && p->sh.dkern == 3 && p->sh.minS < 4) ? 2 : p->sh.dkern;
+    const int kern = (half && p->sh.dkern == 4 && p->sh.minS < 4) ? 3 : p->sh.dkern;
 #define X(a, b)                                                                        \
     if (p->sh.tw == a && p->sh.th == b)                                                \
         return half ? degrid_tiled_t<a, b, true, false>(ctx, src, kern)                \

@@ -193,9 +193,11 @@ typedef struct {
                         0 = one queue; -1 = auto */
     int kernel;      /* gridder: 1 tile in registers, lanes = tile columns; 2 tile in
                         LDS, lanes walk each footprint's points densely; 3 registers,
-                        two columns per lane (16-byte C loads); 0 = auto */
+                        two columns per lane (two taps per load: 16 bytes in FP32);
+                        4 as 3, and four columns per lane in FP16 storage; 0 = auto */
     int degrid_kernel; /* degridder: 1 tile in registers; 2 tile in LDS, dense lanes;
-                        3 LDS with tap pairs (16-byte C loads); 0 = auto */
+                        3 LDS with tap pairs; 4 as 3, and four taps per load in FP16
+                        storage; 0 = auto */
     int grid_phases; /* C-slice phases of the gridder's work items (the degridder uses
                         phases; both cut the same sorted entries); 0 = auto */
 } tcv_tuning;

@@ -30,7 +30,7 @@ def _half_run(ctx, vis, data, C, nPlanes, planeTaps, gSize, planeStart=None, ker
     return g.download(), out.download(), Ch
 
 
-@pytest.mark.parametrize("kernel", [1, 2, 3])
+@pytest.mark.parametrize("kernel", [1, 2, 3, 4])
 def test_half_small_complex_vs_fp64(ctx, kernel):
     Sz, gSize, os_, planes = 65, 320, 8, 5
     iu, iv, co, data, C = small_problem(61, 4000, gSize, Sz, os_=os_, nPlanes=planes)
@@ -95,9 +95,9 @@ def test_half_config1_errors_reported(ctx):
 @pytest.mark.parametrize("Sz", [3, 5, 33])
 @pytest.mark.parametrize("tw,th", [(64, 64), (32, 32), (16, 32), (8, 64)])
 def test_half_four_tap_kernels_shapes(ctx, Sz, tw, th):
-    """FP16 storage with the four-tap kernels at every tile shape (shapes too
-    narrow for four columns per lane, and kernels narrower than four taps,
-    fall back to the other kernels): within the FP16 tolerance of FP64."""
+    """FP16 storage with the four-tap kernels (code 4) at every tile shape
+    (shapes too narrow for four columns per lane, and kernels narrower than
+    four taps, fall back to the two-tap kernels): within the FP16 tolerance."""
     gSize, os_, planes = 192, 4, 3
     iu, iv, co, data, C = small_problem(700 + Sz + tw, 1500, gSize, Sz, os_=os_, nPlanes=planes,
                                         cluster=50)
@@ -105,7 +105,7 @@ def test_half_four_tap_kernels_shapes(ctx, Sz, tw, th):
     d_C, d_data = ctx.to_device(C), ctx.to_device(data)
     Ch = tcv.HalfTable(ctx, d_C, planes, Sz * Sz * os_ * os_)
     dh = tcv.HalfVis(ctx, d_data)
-    plan = tcv.Plan(ctx, vis, tile_w=tw, tile_h=th, kernel=3, degrid_kernel=3, phases=2,
+    plan = tcv.Plan(ctx, vis, tile_w=tw, tile_h=th, kernel=4, degrid_kernel=4, phases=2,
                     slice_keys=8, chunk=64)
     g = ctx.zeros((gSize, gSize), np.complex64)
     plan.grid_half(dh, Ch, g)

@@ -379,7 +379,7 @@ def test_tuning_kernel_field_validated(ctx):
     iu, iv, co, data, C = small_problem(78, 50, gSize, Sz)
     vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
     with pytest.raises(tcv.TcvError):
-        tcv.Plan(ctx, vis, kernel=4)
+        tcv.Plan(ctx, vis, kernel=5)
     with pytest.raises(tcv.TcvError):
         tcv.Plan(ctx, vis, degrid_kernel=7)
     info = tcv.Plan(ctx, vis, kernel=2, degrid_kernel=3, slice_keys=64, phases=16,
