@@ -880,6 +880,7 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled2(ItemSource<BATCH> src)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lx = 2 * (lane % LPR), ly = lane / LPR;
     __shared__ int s_item;
+    __shared__ float s_stage[NW][RPI * 2 * TW];   // one staged row group per wave
     const int nItems = src.total();
     const int q0 = wave * R;
 
@@ -971,18 +972,29 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled2(ItemSource<BATCH> src)
                 }
             }
         } else {
+            // Atomic flush with every instruction on contiguous floats: each row
+            // group goes through this wave's LDS staging rows so that lane l adds
+            // float l (and l + 64, ...) of a tile row, instead of four floats at
+            // a 16-byte stride per lane: four times fewer partly used cache lines
+            // reach the atomic units.
+            float *st = s_stage[wave];
+            float *rowBase = reinterpret_cast<float *>(jd.grid + (size_t)(By + q0 * RPI) * gSize + Bx);
+            const int rowsHere = gSize - (By + q0 * RPI);
+            const int floatsHere = 2 * (gSize - Bx);
 #pragma unroll
             for (int j = 0; j < R; ++j) {
-                float *fp = reinterpret_cast<float *>(g0 + (size_t)j * RPI * gSize);
-                const bool rok = j * RPI < rowsLeft;
-                if (rok && c0ok && (ar0[j] != 0.f || ai0[j] != 0.f)) {
-                    unsafeAtomicAdd(fp, ar0[j]);
-                    unsafeAtomicAdd(fp + 1, ai0[j]);
+                *reinterpret_cast<float4 *>(st + ly * 2 * TW + 2 * lx) =
+                    make_float4(ar0[j], ai0[j], ar1[j], ai1[j]);
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int k = 0; k < RPI * 2 * TW / 64; ++k) {
+                    const int f = k * 64 + lane;              // float within the staged rows
+                    const int q = f / (2 * TW), fc = f % (2 * TW);
+                    const float v = st[f];
+                    if (v != 0.f && j * RPI + q < rowsHere && fc < floatsHere)
+                        unsafeAtomicAdd(rowBase + (size_t)(j * RPI + q) * gSize * 2 + fc, v);
                 }
-                if (rok && c1ok && (ar1[j] != 0.f || ai1[j] != 0.f)) {
-                    unsafeAtomicAdd(fp + 2, ar1[j]);
-                    unsafeAtomicAdd(fp + 3, ai1[j]);
-                }
+                __builtin_amdgcn_wave_barrier();
             }
         }
     }
