@@ -408,10 +408,10 @@ tcv_status tcv_range_errors(tcv_ctx *ctx, long long *count)
 // phases.  Kernels: kDefaultGridKernel / kDefaultDegridKernel.
 // C-slice phases: about `bytesPerPhase` of taps per phase, so that the slices
 // one phase touches stay in L2 / Infinity Cache while its tiles run (measured
-// best: ~5 MB for the degridder, ~6.5 MB for the gridder, whose items also pay
-// a tile flush each), but no more than leaves about 64 entries per (tile,
-// phase) work item on average.  From 8 phases up a multiple of 8, so the
-// phases split evenly over the XCDs.
+// best: ~5.5 MB for both kernels), but no more than leaves about 64 entries
+// per (tile, phase) work item on average: every item zeroes and flushes its
+// tile.  From 8 phases up a multiple of 8, so the phases split evenly over
+// the XCDs.
 static int default_phases(long long cSize, int sliceKeys, const tcv_geometry *g, int maxS, int tw,
                           int th, double bytesPerPhase)
 {
@@ -431,8 +431,8 @@ static int default_phases(long long cSize, int sliceKeys, const tcv_geometry *g,
     return ph;
 }
 
-constexpr double kDegridPhaseBytes = 5.0e6;
-constexpr double kGridPhaseBytes = 6.5e6;
+constexpr double kDegridPhaseBytes = 5.5e6;
+constexpr double kGridPhaseBytes = 5.5e6;   // the gridder may use its own count (grid_phases)
 
 static void auto_tuning(int maxS, const tcv_geometry *g, tcv_tuning *t)
 {
