@@ -1128,7 +1128,7 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled4h(ItemSource<BATCH> src)
 // every cell accumulates the item's entries in entry order, rounded exactly as
 // the register-tile gridder rounds them.
 #ifndef TCV_LDS_U
-#define TCV_LDS_U 4
+#define TCV_LDS_U 8
 #endif
 constexpr int kLdsU = TCV_LDS_U;   // footprint points per lane whose taps load together
 
