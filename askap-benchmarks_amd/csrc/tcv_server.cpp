@@ -204,9 +204,15 @@ void scheduler(tcv_server *s)
             std::vector<Req *> batch;
             {
                 std::lock_guard<std::mutex> lk(s->mu);
+                long long points = 0;
                 while (!s->admitted.empty() &&
                        (s->cfg.maxBatch <= 0 || (int)batch.size() < s->cfg.maxBatch)) {
                     Req *r = s->reqs[s->admitted.front()].get();
+                    // a batch completes as one: keep large requests apart from small ones
+                    if (!batch.empty() && s->cfg.maxBatchPoints > 0 &&
+                        points + r->points > s->cfg.maxBatchPoints)
+                        break;
+                    points += r->points;
                     s->admitted.pop_front();
                     r->state = RUNNING;
                     r->tLaunch = s->now();
@@ -258,7 +264,8 @@ extern "C" {
 
 tcv_status tcv_server_create(tcv_ctx *ctx, const tcv_server_config *cfg, tcv_server **out)
 {
-    if (!ctx || !cfg || !out || cfg->arenaBytes < 4096 || cfg->maxInflight < 1 || cfg->maxInflight > 8)
+    if (!ctx || !cfg || !out || cfg->arenaBytes < 4096 || cfg->maxInflight < 1 || cfg->maxInflight > 8 ||
+        cfg->maxBatch < 0 || cfg->maxBatchPoints < 0)
         return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_server_create: bad argument");
     *out = nullptr;
     auto s = std::make_unique<tcv_server>();

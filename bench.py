@@ -173,75 +173,85 @@ def bench_major(ctx, args):
     return out
 
 
-def bench_serving(ctx, args, capacity):
-    """Poisson request streams against the continuous-batching server.
+def serving_stream(srv, bj, mean_pts, capacity, rate, n, seed):
+    """One Poisson stream of n requests at `rate` per second against `srv`:
+    requests drawn from the resident jobs, 50% grid into a new grid, 50%
+    degrid the job's resident grid.  Latency = submission to completion."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    ops = rng.integers(0, 2, n)
+    tmpl = rng.integers(0, len(bj.jobs), n)
+    t0 = srv.clock() + 0.05
+    ids = []
+    for i in range(n):
+        target = t0 + arrivals[i]
+        while True:
+            d = target - srv.clock()
+            if d <= 0:
+                break
+            time.sleep(min(d, 0.002) if d > 0.0005 else 0)
+        j = bj.jobs[tmpl[i]]
+        if ops[i] == 0:
+            ids.append(srv.submit(0, j["vis"], j["C"], data=j["data"]))
+        else:
+            ids.append(srv.submit(1, j["vis"], j["C"], grid=j["grid"]))
+    res = []
+    for r in ids:
+        res.append(srv.wait(r, timeout=900))
+        srv.release(r)
+    bad = sum(r["status"] != 0 for r in res)
+    lat = np.array([r["completed"] - r["submitted"] for r in res]) * 1e3
+    wait = np.array([r["launched"] - r["submitted"] for r in res]) * 1e3
+    pts = float(sum(r["points"] for r in res))
+    span = max(r["completed"] for r in res) - min(r["submitted"] for r in res)
+    offered = rate * mean_pts
+    return {"rate_req_s": rate, "requests": n, "failed": bad,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+            "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)),
+            "queue_wait_p50_ms": float(np.percentile(wait, 50)),
+            "throughput": pts / span, "offered_points_per_s": offered,
+            "utilisation": offered / capacity}
 
-    Requests are drawn from the 64 resident uneven-batch jobs (so sizes follow
-    that distribution): 50% grid into a new grid, 50% degrid the job's
-    resident grid.  Admission is by the HBM arena, sized so that resident data
-    plus the arena stay within a 200 GB budget.  Two streams: the specified
-    rate, and a declared rate at 60% of the measured batched capacity."""
+
+def serving_setup(ctx):
+    """The 64 uneven jobs resident on the device (grids gridded once for the
+    degrid requests) and the arena that keeps resident data plus requests
+    within a 200 GB budget."""
     from tcv import setup as S
     from tcv.gridder import BatchJobs
     specs = S.config3(nJobs=64)
     bj = BatchJobs(ctx, specs)
     bj.zero_grids()
-    bj.run_batched(graph=False)          # resident grids for the degrid requests
+    bj.run_batched(graph=False)
     ctx.sync()
     free, total = ctx.mem_info()
     resident = total - free
     budget = 200 * 10**9
     arena = int(min(budget - resident, free - 16 * 2**30))
-    import tcv
-    srv = tcv.Server(ctx, arena, max_inflight=3, auto_release=True)
     mean_pts = float(np.mean([sp["nVis"] * sp["sSize"] ** 2 for sp in specs]))
+    return bj, arena, resident, budget, mean_pts
 
-    def stream(rate, n, seed):
-        rng = np.random.default_rng(seed)
-        arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
-        ops = rng.integers(0, 2, n)
-        tmpl = rng.integers(0, len(bj.jobs), n)
-        t0 = srv.clock() + 0.05
-        ids = []
-        for i in range(n):
-            target = t0 + arrivals[i]
-            while True:
-                d = target - srv.clock()
-                if d <= 0:
-                    break
-                time.sleep(min(d, 0.002) if d > 0.0005 else 0)
-            j = bj.jobs[tmpl[i]]
-            if ops[i] == 0:
-                ids.append(srv.submit(0, j["vis"], j["C"], data=j["data"]))
-            else:
-                ids.append(srv.submit(1, j["vis"], j["C"], grid=j["grid"]))
-        res = []
-        for r in ids:
-            res.append(srv.wait(r, timeout=900))
-            srv.release(r)
-        bad = sum(r["status"] != 0 for r in res)
-        lat = np.array([r["completed"] - r["submitted"] for r in res]) * 1e3
-        wait = np.array([r["launched"] - r["submitted"] for r in res]) * 1e3
-        pts = float(sum(r["points"] for r in res))
-        span = max(r["completed"] for r in res) - min(r["submitted"] for r in res)
-        offered = rate * mean_pts
-        return {"rate_req_s": rate, "requests": n, "failed": bad,
-                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
-                "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)),
-                "queue_wait_p50_ms": float(np.percentile(wait, 50)),
-                "throughput": pts / span, "offered_points_per_s": offered,
-                "utilisation": offered / capacity}
 
+def bench_serving(ctx, args, capacity):
+    """Poisson request streams against the continuous-batching server: the
+    specified rate, and a declared rate at 60% of the measured batched
+    capacity.  Requests follow the uneven 64-job size distribution."""
+    import tcv
+    bj, arena, resident, budget, mean_pts = serving_setup(ctx)
+    srv = tcv.Server(ctx, arena, max_inflight=args.serve_inflight, auto_release=True,
+                     max_batch_points=int(args.serve_batch_points))
     out = {"workload": "Poisson requests over the 64 uneven jobs, 50% grid into new grids, 50% degrid "
-                       "resident grids, HBM arena admission, 3 overlapping batches",
+                       "resident grids, HBM arena admission, overlapping batches",
            "arena_bytes": arena, "resident_bytes": resident, "budget_bytes": budget,
+           "max_inflight": args.serve_inflight, "max_batch_points": args.serve_batch_points,
            "capacity_points_per_s": capacity, "mean_points_per_request": mean_pts}
-    out["specified"] = stream(200.0, args.serve_requests, 1)
+    out["specified"] = serving_stream(srv, bj, mean_pts, capacity, 200.0, args.serve_requests, 1)
     log(f"serving @200/s: p50 {out['specified']['p50_ms']:.1f} ms p99 {out['specified']['p99_ms']:.1f} ms, "
         f"utilisation {out['specified']['utilisation']:.2f}")
     rate = 0.6 * capacity / mean_pts
-    out["declared_60pct"] = stream(rate, max(200, args.serve_requests // 2), 2)
+    out["declared_60pct"] = serving_stream(srv, bj, mean_pts, capacity, rate,
+                                           max(200, args.serve_requests // 2), 2)
     log(f"serving @{rate:.0f}/s: p50 {out['declared_60pct']['p50_ms']:.1f} ms "
         f"p99 {out['declared_60pct']['p99_ms']:.1f} ms")
     srv.close()
@@ -276,6 +286,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-inflight", type=int, default=3)
+    ap.add_argument("--serve-batch-points", type=float, default=0.0)
     ap.add_argument("--no-major", action="store_true")
     ap.add_argument("--major-samples", type=int, default=160000)
     ap.add_argument("--major-cycles", type=int, default=5)

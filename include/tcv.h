@@ -334,6 +334,8 @@ typedef struct {
     int maxBatch;        /* requests per batch (0 = all admitted) */
     int tile_w, tile_h, chunk, slice_keys, phases;   /* plan tuning (0 = auto) */
     int autoRelease;     /* 1: free each result when its request completes */
+    long long maxBatchPoints; /* a batch closes once it holds this many grid points, so
+                                 large requests run apart from small ones (0 = no limit) */
 } tcv_server_config;
 
 typedef struct {

@@ -114,3 +114,54 @@ def test_admission_waits_for_memory_and_cancel(ctx, templates):
     if cancelled:
         assert res[-1]["status"] == 7     # TCV_ERR_CANCELLED
     srv.close()
+
+
+def _batches_one_slot(res):
+    """With one slot, batches run one after another: a request opens a new
+    batch when it launched after the running batch completed."""
+    batches, done = [], -1.0
+    for r in res:
+        if r["launched"] >= done:
+            batches.append([])
+            done = r["completed"]
+        batches[-1].append(r)
+    return batches
+
+
+@pytest.mark.parametrize("limit_of", ["below_every_request", "twice_the_median"])
+def test_batch_points_limit(ctx, templates, limit_of):
+    """maxBatchPoints: a batch takes admitted requests in order until the next
+    would carry it past the limit (a lone request always runs); results are
+    unchanged."""
+    jobs = templates.jobs
+    pts = sorted(j["spec"]["nVis"] * j["spec"]["sSize"] ** 2 for j in jobs)
+    limit = 1 if limit_of == "below_every_request" else 2 * pts[len(pts) // 2]
+    srv = tcv.Server(ctx, 8 * GB, max_inflight=1, max_batch_points=limit)
+    order = [(j, op) for op in (0, 1) for j in range(len(jobs))] * 2
+    ids = [(submit(srv, jobs[j], op), j, op) for j, op in order]
+    res = []
+    for rid, j, op in ids:
+        r, out = result_of(srv, rid, jobs[j], op)
+        res.append(r)
+        sp = jobs[j]["spec"]
+        if op == 0:
+            want = ctx.zeros((sp["gSize"], sp["gSize"]), np.complex64)
+            jobs[j]["plan"].grid(jobs[j]["data"], jobs[j]["C"], want)
+        else:
+            want = ctx.empty(sp["nVis"], np.complex64)
+            jobs[j]["plan"].degrid(jobs[j]["grid"], jobs[j]["C"], want)
+        assert rel_err(out, want.download()) <= TOL, (j, op)
+        srv.release(rid)
+    batches = _batches_one_slot(res)
+    for b in batches:
+        assert len(b) == 1 or sum(r["points"] for r in b) <= limit
+    if limit == 1:
+        assert len(batches) == len(order)
+    srv.close()
+
+
+def test_server_config_rejects_negative_limits(ctx):
+    with pytest.raises(tcv.TcvError):
+        tcv.Server(ctx, GB, max_batch_points=-1)
+    with pytest.raises(tcv.TcvError):
+        tcv.Server(ctx, GB, max_batch=-2)
