@@ -286,8 +286,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--serve-inflight", type=int, default=3)
-    ap.add_argument("--serve-batch-points", type=float, default=0.0)
+    ap.add_argument("--serve-inflight", type=int, default=4)
+    ap.add_argument("--serve-batch-points", type=float, default=1e10)
     ap.add_argument("--no-major", action="store_true")
     ap.add_argument("--major-samples", type=int, default=160000)
     ap.add_argument("--major-cycles", type=int, default=5)
