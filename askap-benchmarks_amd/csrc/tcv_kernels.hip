@@ -27,6 +27,38 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, unsig
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, bytes, 0x00020000);
 }
 
+// Load of wave-uniform data that no wave writes while the kernel runs (work
+// items, entries, visibilities, plane scales).  Through the constant address
+// space the compiler issues a scalar (SMEM) load into SGPRs, instead of a
+// vector load, a wait for it and readfirstlane.
+template <typename T>
+__device__ __forceinline__ T cload(const T *p, int i)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(4))) T *)p)[i];
+#else
+    return p[i];   // the host pass only parses device functions
+#endif
+}
+
+// a * b for 0 <= a, b < 2^24, b wave-uniform: the full-rate 24-bit multiply,
+// written as the instruction so that the compiler cannot fold it into a
+// neighbouring expression as a quarter-rate 32- or 64-bit multiply
+__device__ __forceinline__ int umul24(int a, int b)
+{
+    int r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "s"(b), "v"(a));
+    return r;
+}
+
+// a * b + c (24-bit unsigned a, b; b wave-uniform)
+__device__ __forceinline__ unsigned umad24(int a, int b, unsigned c)
+{
+    unsigned r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+    return r;
+}
+
 __device__ __forceinline__ float2 load_c(__amdgpu_buffer_rsrc_t r, unsigned voff)
 {
     return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
@@ -579,10 +611,10 @@ __device__ __forceinline__ float2 vis_value(const JobDesc &d, const int4 &en)
 {
     if constexpr (!HALF) {
         if (!d.data) return make_float2(1.f, 0.f);   // PSF gridding: data == 1 + 0i
-        return static_cast<const float2 *>(d.data)[en.z];
+        return cload(static_cast<const float2 *>(d.data), en.z);
     } else {
-        const half2_t v = static_cast<const half2_t *>(d.data)[en.z];
-        const float sc = d.dataScale * d.planeScale[entry_plane(en, d)];
+        const half2_t v = cload(static_cast<const half2_t *>(d.data), en.z);
+        const float sc = d.dataScale * cload(d.planeScale, entry_plane(en, d));
         return make_float2((float)v.x * sc, (float)v.y * sc);
     }
 }
@@ -691,7 +723,7 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled(ItemSource<BATCH> src)
         const __amdgpu_buffer_rsrc_t rc = make_rsrc(jd.C, jd.cBytes);
         const int4 *__restrict__ entries = jd.entries;
         const int gSize = jd.gSize;
-        const int4 it = jd.items[local];
+        const int4 it = cload(jd.items, local);
         const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
         const int Bx = txi * TW, By = tyi * TH;
         const int e0 = it.y, e1 = it.y + it.z;
@@ -700,12 +732,12 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled(ItemSource<BATCH> src)
 #pragma unroll
         for (int j = 0; j < R; ++j) { ar[j] = 0.f; ai[j] = 0.f; }
 
-        int4 enA = entries[e0];
-        int4 enB = entries[min(e0 + 1, e1 - 1)];
+        int4 enA = cload(entries, e0);
+        int4 enB = cload(entries, min(e0 + 1, e1 - 1));
         float2 dA = vis_value<HALF>(jd, enA);
         for (int e = e0; e < e1; ++e) {
             const float2 dB = vis_value<HALF>(jd, enB);
-            const int4 enC = entries[min(e + 2, e1 - 1)];
+            const int4 enC = cload(entries, min(e + 2, e1 - 1));
 
             const int u0 = enA.x & 0xFFFF, v0 = (int)((unsigned)enA.x >> 16);
             const int c0 = enA.y, s = entry_width(enA.w);
@@ -798,7 +830,7 @@ __global__ __launch_bounds__(NW * 64) void k_degrid_tiled(ItemSource<BATCH> src)
         const int4 *__restrict__ entries = jd.entries;
         const float2 *__restrict__ grid = jd.grid;
         const int gSize = jd.gSize;
-        const int4 it = jd.items[local];
+        const int4 it = cload(jd.items, local);
         const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
         const int Bx = txi * TW, By = tyi * TH;
         const int e0 = it.y, e1 = it.y + it.z;
@@ -814,9 +846,9 @@ __global__ __launch_bounds__(NW * 64) void k_degrid_tiled(ItemSource<BATCH> src)
 
         const int first = e0 + wave;
         if (first >= e1) continue;
-        int4 enA = entries[first];
+        int4 enA = cload(entries, first);
         for (int e = first; e < e1; e += NW) {
-            const int4 enB = entries[min(e + NW, e1 - 1)];
+            const int4 enB = cload(entries, min(e + NW, e1 - 1));
             const int u0 = enA.x & 0xFFFF, v0 = (int)((unsigned)enA.x >> 16);
             const int c0 = enA.y, s = entry_width(enA.w);
             const int cx = Bx + lx - u0;
@@ -846,7 +878,7 @@ __global__ __launch_bounds__(NW * 64) void k_degrid_tiled(ItemSource<BATCH> src)
             sr = wave_sum(sr);
             si = wave_sum(si);
             if constexpr (HALF) {
-                const float ps = jd.planeScale[entry_plane(enA, jd)];
+                const float ps = cload(jd.planeScale, entry_plane(enA, jd));
                 sr *= ps;
                 si *= ps;
             }
@@ -894,9 +926,10 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled2(ItemSource<BATCH> src)
         int local;
         src.locate(item, jd, local);
         const __amdgpu_buffer_rsrc_t rc = make_rsrc(jd.C, jd.cBytes);
+        const int nTaps = (int)(jd.cBytes / TB);
         const int4 *__restrict__ entries = jd.entries;
         const int gSize = jd.gSize;
-        const int4 it = jd.items[local];
+        const int4 it = cload(jd.items, local);
         const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
         const int Bx = txi * TW, By = tyi * TH;
         const int e0 = it.y, e1 = it.y + it.z;
@@ -905,45 +938,76 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled2(ItemSource<BATCH> src)
 #pragma unroll
         for (int j = 0; j < R; ++j) { ar0[j] = ai0[j] = ar1[j] = ai1[j] = 0.f; }
 
-        int4 enA = entries[e0];
-        int4 enB = entries[min(e0 + 1, e1 - 1)];
+        int4 enA = cload(entries, e0);
+        int4 enB = cload(entries, min(e0 + 1, e1 - 1));
         float2 dA = vis_value<HALF>(jd, enA);
         for (int e = e0; e < e1; ++e) {
             const float2 dB = vis_value<HALF>(jd, enB);
-            const int4 enC = entries[min(e + 2, e1 - 1)];
+            const int4 enC = cload(entries, min(e + 2, e1 - 1));
             const int u0 = enA.x & 0xFFFF, v0 = (int)((unsigned)enA.x >> 16);
             const int c0 = enA.y, s = entry_width(enA.w);
             const int cx = Bx + lx - u0;   // C column of the lane's first cell
-            // pair start and which half feeds which cell
             const bool in0 = (unsigned)cx < (unsigned)s, in1 = (unsigned)(cx + 1) < (unsigned)s;
-            const int start = min(max(cx, 0), s - 2);
-            const bool sh = cx > start;    // right edge: the wanted tap is the pair's second
-            const unsigned lanev = (in0 || in1) ? (unsigned)(c0 + start) * TB : kOOB;
             const int dy = By - v0;
+            const int cyl = ly + dy;       // C row of the lane's row in group 0, less r0
+            const int sTB = s * (int)TB;
+            if (c0 > 0 && c0 + s * s < nTaps) {
+                // the pair (cx, cx + 1) even where it straddles the footprint's edge:
+                // the outside tap is the neighbouring one in the table, and its cell
+                // takes it with a zero visibility (an exact zero, tables are finite)
+                const float2 d0 = in0 ? dA : make_float2(0.f, 0.f);
+                const float2 d1 = in1 ? dA : make_float2(0.f, 0.f);
+                const unsigned lanev = (in0 || in1) ? (unsigned)(c0 + cx) * TB : kOOB;
 #pragma unroll
-            for (int jb = 0; jb < R; jb += RB) {
-                const int r0 = (q0 + jb) * RPI;
-                if (r0 + RB * RPI + dy > 0 && r0 + dy < s) {
-                    float4 cv[RB];
+                for (int jb = 0; jb < R; jb += RB) {
+                    const int r0 = (q0 + jb) * RPI;
+                    if (r0 + RB * RPI + dy > 0 && r0 + dy < s) {
+                        float4 cv[RB];
 #pragma unroll
-                    for (int k = 0; k < RB; ++k) {
-                        const int cy = r0 + k * RPI + ly + dy;
-                        const unsigned vo = ((unsigned)cy < (unsigned)s)
-                                                ? lanev + (unsigned)(cy * s) * TB : kOOB;
-                        cv[k] = load_pair<HALF>(rc, vo);
+                        for (int k = 0; k < RB; ++k) {
+                            const int cy = cyl + r0 + k * RPI;
+                            cv[k] = load_pair<HALF>(
+                                rc, (unsigned)cy < (unsigned)s ? umad24(cy, sTB, lanev) : kOOB);
+                        }
+#pragma unroll
+                        for (int k = 0; k < RB; ++k) {
+                            ar0[jb + k] = fmaf(d0.x, cv[k].x, fmaf(-d0.y, cv[k].y, ar0[jb + k]));
+                            ai0[jb + k] = fmaf(d0.x, cv[k].y, fmaf(d0.y, cv[k].x, ai0[jb + k]));
+                            ar1[jb + k] = fmaf(d1.x, cv[k].z, fmaf(-d1.y, cv[k].w, ar1[jb + k]));
+                            ai1[jb + k] = fmaf(d1.x, cv[k].w, fmaf(d1.y, cv[k].z, ai1[jb + k]));
+                        }
                     }
+                }
+            } else {
+                // slice at the table's start or end: load the pair that starts (or
+                // ends) on the edge tap, inside the slice, and pick the halves
+                const int start = min(max(cx, 0), s - 2);
+                const bool sh = cx > start;    // right edge: the wanted tap is the pair's second
+                const unsigned lanev = (in0 || in1) ? (unsigned)(c0 + start) * TB : kOOB;
 #pragma unroll
-                    for (int k = 0; k < RB; ++k) {
-                        // cell 0 takes tap 0 (tap 1 at the right edge); cell 1 takes
-                        // tap 1 (tap 0 when the pair starts one column early)
-                        float2 t0 = sh ? make_float2(cv[k].z, cv[k].w) : make_float2(cv[k].x, cv[k].y);
-                        float2 t1 = in0 ? make_float2(cv[k].z, cv[k].w) : make_float2(cv[k].x, cv[k].y);
-                        if (!in0) t0 = make_float2(0.f, 0.f);
-                        if (!in1) t1 = make_float2(0.f, 0.f);
-                        ar0[jb + k] = fmaf(dA.x, t0.x, fmaf(-dA.y, t0.y, ar0[jb + k]));
-                        ai0[jb + k] = fmaf(dA.x, t0.y, fmaf(dA.y, t0.x, ai0[jb + k]));
-                        ar1[jb + k] = fmaf(dA.x, t1.x, fmaf(-dA.y, t1.y, ar1[jb + k]));
-                        ai1[jb + k] = fmaf(dA.x, t1.y, fmaf(dA.y, t1.x, ai1[jb + k]));
+                for (int jb = 0; jb < R; jb += RB) {
+                    const int r0 = (q0 + jb) * RPI;
+                    if (r0 + RB * RPI + dy > 0 && r0 + dy < s) {
+                        float4 cv[RB];
+#pragma unroll
+                        for (int k = 0; k < RB; ++k) {
+                            const int cy = cyl + r0 + k * RPI;
+                            cv[k] = load_pair<HALF>(
+                                rc, (unsigned)cy < (unsigned)s ? umad24(cy, sTB, lanev) : kOOB);
+                        }
+#pragma unroll
+                        for (int k = 0; k < RB; ++k) {
+                            // cell 0 takes tap 0 (tap 1 at the right edge); cell 1 takes
+                            // tap 1 (tap 0 when the pair starts one column late)
+                            float2 t0 = sh ? make_float2(cv[k].z, cv[k].w) : make_float2(cv[k].x, cv[k].y);
+                            float2 t1 = in0 ? make_float2(cv[k].z, cv[k].w) : make_float2(cv[k].x, cv[k].y);
+                            if (!in0) t0 = make_float2(0.f, 0.f);
+                            if (!in1) t1 = make_float2(0.f, 0.f);
+                            ar0[jb + k] = fmaf(dA.x, t0.x, fmaf(-dA.y, t0.y, ar0[jb + k]));
+                            ai0[jb + k] = fmaf(dA.x, t0.y, fmaf(dA.y, t0.x, ai0[jb + k]));
+                            ar1[jb + k] = fmaf(dA.x, t1.x, fmaf(-dA.y, t1.y, ar1[jb + k]));
+                            ai1[jb + k] = fmaf(dA.x, t1.y, fmaf(dA.y, t1.x, ai1[jb + k]));
+                        }
                     }
                 }
             }
@@ -1034,7 +1098,7 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled4h(ItemSource<BATCH> src)
         const __amdgpu_buffer_rsrc_t rc = make_rsrc(jd.C, jd.cBytes);
         const int4 *__restrict__ entries = jd.entries;
         const int gSize = jd.gSize;
-        const int4 it = jd.items[local];
+        const int4 it = cload(jd.items, local);
         const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
         const int Bx = txi * TW, By = tyi * TH;
         const int e0 = it.y, e1 = it.y + it.z;
@@ -1045,12 +1109,12 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled4h(ItemSource<BATCH> src)
 #pragma unroll
             for (int c = 0; c < V; ++c) ar[j][c] = ai[j][c] = 0.f;
 
-        int4 enA = entries[e0];
-        int4 enB = entries[min(e0 + 1, e1 - 1)];
+        int4 enA = cload(entries, e0);
+        int4 enB = cload(entries, min(e0 + 1, e1 - 1));
         float2 dA = vis_value<true>(jd, enA);
         for (int e = e0; e < e1; ++e) {
             const float2 dB = vis_value<true>(jd, enB);
-            const int4 enC = entries[min(e + 2, e1 - 1)];
+            const int4 enC = cload(entries, min(e + 2, e1 - 1));
             const int u0 = enA.x & 0xFFFF, v0 = (int)((unsigned)enA.x >> 16);
             const int c0 = enA.y, s = entry_width(enA.w);
             const int cx = Bx + lx - u0;               // C column of the lane's first cell
@@ -1137,7 +1201,7 @@ __device__ __forceinline__ void split_flat(int p, float rw, int w, int &row, int
     // p < 2^12, w <= 64: (p + 0.5) / w is at least 1/(2w) away from an integer,
     // far beyond the rounding of the float reciprocal
     row = (int)(((float)p + 0.5f) * rw);
-    col = p - row * w;
+    col = p - umul24(row, w);
 }
 
 template <int TW, int TH, bool HALF, bool BATCH>
@@ -1160,7 +1224,7 @@ __global__ __launch_bounds__(256) void k_grid_lds(ItemSource<BATCH> src)
         const __amdgpu_buffer_rsrc_t rc = make_rsrc(jd.C, jd.cBytes);
         const int4 *__restrict__ entries = jd.entries;
         const int gSize = jd.gSize;
-        const int4 it = jd.items[local];
+        const int4 it = cload(jd.items, local);
         const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
         const int Bx = txi * TW, By = tyi * TH;
 
@@ -1169,7 +1233,7 @@ __global__ __launch_bounds__(256) void k_grid_lds(ItemSource<BATCH> src)
             tile[(wave + 4 * (q / TW)) * TW + (q % TW)] = make_float2(0.f, 0.f);
 
         for (int e = it.y; e < it.y + it.z; ++e) {
-            const int4 en = entries[e];
+            const int4 en = cload(entries, e);
             const float2 d = vis_value<HALF>(jd, en);
             const int u0 = en.x & 0xFFFF, v0 = (int)((unsigned)en.x >> 16);
             const int c0 = en.y, s = entry_width(en.w);
@@ -1251,7 +1315,7 @@ __global__ __launch_bounds__(256) void k_degrid_lds(ItemSource<BATCH> src)
         const int4 *__restrict__ entries = jd.entries;
         const float2 *__restrict__ grid = jd.grid;
         const int gSize = jd.gSize;
-        const int4 it = jd.items[local];
+        const int4 it = cload(jd.items, local);
         const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
         const int Bx = txi * TW, By = tyi * TH;
         for (int q = threadIdx.x; q < TH * TW; q += 256) {
@@ -1262,7 +1326,7 @@ __global__ __launch_bounds__(256) void k_degrid_lds(ItemSource<BATCH> src)
         __syncthreads();
 
         for (int e = it.y + wave; e < it.y + it.z; e += 4) {
-            const int4 en = entries[e];
+            const int4 en = cload(entries, e);
             const int u0 = en.x & 0xFFFF, v0 = (int)((unsigned)en.x >> 16);
             const int c0 = en.y, s = entry_width(en.w);
             const int colLo = max(u0 - Bx, 0), colHi = min(u0 + s - Bx, TW);
@@ -1293,7 +1357,7 @@ __global__ __launch_bounds__(256) void k_degrid_lds(ItemSource<BATCH> src)
             sr = wave_sum(sr);
             si = wave_sum(si);
             if constexpr (HALF) {
-                const float ps = jd.planeScale[entry_plane(en, jd)];
+                const float ps = cload(jd.planeScale, entry_plane(en, jd));
                 sr *= ps;
                 si *= ps;
             }
@@ -1307,13 +1371,16 @@ __global__ __launch_bounds__(256) void k_degrid_lds(ItemSource<BATCH> src)
 }
 
 // Degridder with the LDS tile and tap pairs: lanes walk each entry's
-// footprint on the tile two columns at a time (16-byte C loads, two-cell LDS
-// reads).  The last pair of a row with an odd width uses one half; a pair that
-// would end past the C row starts one tap earlier instead.
+// footprint on the tile densely, one pair of adjacent cells per lane and step
+// (a 16-byte C load, two LDS cells); with an odd width the last column follows
+// as one pass with a lane per row.  No pair leaves the footprint, so every tap
+// read is one the reference reads.  Index arithmetic is 24-bit (full-rate),
+// and only the last step of an entry checks for lanes past its end.
 template <int TW, int TH, bool HALF, bool BATCH>
 __global__ __launch_bounds__(256) void k_degrid_lds2(ItemSource<BATCH> src)
 {
     constexpr unsigned TB = Taps<HALF>::kBytes;
+    constexpr unsigned kLastPair = (TH * TW - 2) * 8u;   // byte offset of the tile's last pair
     __shared__ float2 tile[TH * TW];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1331,7 +1398,7 @@ __global__ __launch_bounds__(256) void k_degrid_lds2(ItemSource<BATCH> src)
         const int4 *__restrict__ entries = jd.entries;
         const float2 *__restrict__ grid = jd.grid;
         const int gSize = jd.gSize;
-        const int4 it = jd.items[local];
+        const int4 it = cload(jd.items, local);
         const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
         const int Bx = txi * TW, By = tyi * TH;
         for (int q = threadIdx.x; q < TH * TW; q += 256) {
@@ -1341,44 +1408,55 @@ __global__ __launch_bounds__(256) void k_degrid_lds2(ItemSource<BATCH> src)
         }
         __syncthreads();
 
-        for (int e = it.y + wave; e < it.y + it.z; e += 4) {
-            const int4 en = entries[e];
+        const int eEnd = it.y + it.z;
+        for (int e = it.y + wave; e < eEnd; e += 4) {
+            const int4 en = cload(entries, e);
             const int u0 = en.x & 0xFFFF, v0 = (int)((unsigned)en.x >> 16);
             const int c0 = en.y, s = entry_width(en.w);
             const int colLo = max(u0 - Bx, 0), colHi = min(u0 + s - Bx, TW);
             const int rowLo = max(v0 - By, 0), rowHi = min(v0 + s - By, TH);
             const int w = colHi - colLo, h = rowHi - rowLo;
             if (w <= 0 || h <= 0) continue;
-            const int wp = (w + 1) >> 1;   // pairs per row
+            const int wp = w >> 1;                            // whole pairs per row
             const int n = wp * h;
-            const float rw = 1.0f / (float)wp;
-            const int cxLo = Bx + colLo - u0;                 // C column of colLo
-            const int cRow = c0 + (By + rowLo - v0) * s;      // C offset of the first row
-            const float2 *tBase = tile + rowLo * TW + colLo;
+            const float rw = 1.0f / (float)max(wp, 1);
+            // byte offsets of cell (rowLo, colLo): in C (its tap) and in the tile
+            const unsigned cB = (unsigned)(c0 + (By + rowLo - v0) * s + Bx + colLo - u0) * TB;
+            const unsigned lB = (unsigned)(rowLo * TW + colLo) * 8u;
+            const int sB = s * (int)TB;
             float sr = 0.f, si = 0.f;
-            for (int p0 = lane; p0 < n; p0 += 64 * kLdsU) {
+            // one step: pair k = p - rl * wp of row rl for p = p0 + lane + 64 u.
+            // The offsets of the pair, base + rl * sB + 2 k TB (C) and
+            // base + rl * TW * 8 + 16 k (tile), are written as
+            // (base + 2 p TB) + rl * (sB - 2 wp TB): one 24-bit multiply-add each.
+            const int aC = sB - 2 * (int)TB * wp, aL = TW * 8 - 16 * wp;   // both >= 0
+            auto step = [&](int p0, auto tailTag) {
+                constexpr bool TAIL = decltype(tailTag)::value;
                 float4 c[kLdsU];
                 float2 g0[kLdsU], g1[kLdsU];
+                const int pl = p0 + lane;
+                const float pf = (float)pl + 0.5f;
+                const unsigned cP = cB + (unsigned)pl * (2u * TB), lP = lB + (unsigned)pl * 16u;
 #pragma unroll
                 for (int u = 0; u < kLdsU; ++u) {
-                    const int p = p0 + 64 * u;
-                    int rl, k;
-                    split_flat(p, rw, wp, rl, k);
-                    const int x = 2 * k;                      // first cell, from colLo
-                    const bool two = x + 1 < w;
-                    const int cx = cxLo + x;
-                    const int start = min(cx, s - 2);         // s >= 2 for this kernel
-                    c[u] = load_pair<HALF>(rc, p < n ? (unsigned)(cRow + rl * s + start) * TB : kOOB);
-                    const float2 *t = tBase + rl * TW + x;
-                    const float2 a = p < n ? t[0] : make_float2(0.f, 0.f);
-                    const float2 b = (p < n && two) ? t[1] : make_float2(0.f, 0.f);
-                    // the taps of cells x and x + 1 sit at (cx - start) and +1 in the pair
-                    if (cx > start) {   // pair starts one early: cell x uses the second tap
-                        g0[u] = make_float2(0.f, 0.f);
-                        g1[u] = a;
-                    } else {
-                        g0[u] = a;
-                        g1[u] = b;
+                    // row = floor((p + 0.5) / wp): p + 0.5 is at least 1/(2 wp) from a
+                    // multiple of wp, far beyond the rounding of the float arithmetic
+                    const int rl = (int)fmaf(pf, rw, (float)(64 * u) * rw);
+                    unsigned co = umad24(rl, aC, cP) + (unsigned)(64 * u) * (2u * TB);
+                    unsigned lo = umad24(rl, aL, lP) + (unsigned)(64 * u) * 16u;
+                    bool ok = true;
+                    if constexpr (TAIL) {
+                        ok = pl + 64 * u < n;
+                        co = ok ? co : kOOB;
+                        lo = ok ? lo : kLastPair;
+                    }
+                    c[u] = load_pair<HALF>(rc, co);
+                    const float2 *t = reinterpret_cast<const float2 *>(
+                        reinterpret_cast<const char *>(tile) + lo);
+                    g0[u] = t[0];
+                    g1[u] = t[1];
+                    if constexpr (TAIL) {
+                        if (!ok) g0[u] = g1[u] = make_float2(0.f, 0.f);
                     }
                 }
 #pragma unroll
@@ -1388,11 +1466,25 @@ __global__ __launch_bounds__(256) void k_degrid_lds2(ItemSource<BATCH> src)
                     sr = fmaf(g1[u].x, c[u].z, fmaf(-g1[u].y, c[u].w, sr));
                     si = fmaf(g1[u].x, c[u].w, fmaf(g1[u].y, c[u].z, si));
                 }
+            };
+            int p0 = 0;
+            for (; p0 + 64 * kLdsU <= n; p0 += 64 * kLdsU) step(p0, std::false_type{});
+            if (p0 < n) step(p0, std::true_type{});
+            if (w & 1) {
+                // the last column, a lane per row
+                const bool ok = lane < h;
+                const unsigned col = (unsigned)(w - 1);
+                const float2 cv = Taps<HALF>::load(
+                    rc, ok ? umad24(lane, sB, cB + col * TB) : kOOB);
+                const float2 g = tile[min((rowLo + lane) * TW + colLo + w - 1, TH * TW - 1)];
+                const float2 gz = ok ? g : make_float2(0.f, 0.f);
+                sr = fmaf(gz.x, cv.x, fmaf(-gz.y, cv.y, sr));
+                si = fmaf(gz.x, cv.y, fmaf(gz.y, cv.x, si));
             }
             sr = wave_sum(sr);
             si = wave_sum(si);
             if constexpr (HALF) {
-                const float ps = jd.planeScale[entry_plane(en, jd)];
+                const float ps = cload(jd.planeScale, entry_plane(en, jd));
                 sr *= ps;
                 si *= ps;
             }
@@ -1429,7 +1521,7 @@ __global__ __launch_bounds__(256) void k_degrid_lds4h(ItemSource<BATCH> src)
         const int4 *__restrict__ entries = jd.entries;
         const float2 *__restrict__ grid = jd.grid;
         const int gSize = jd.gSize;
-        const int4 it = jd.items[local];
+        const int4 it = cload(jd.items, local);
         const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
         const int Bx = txi * TW, By = tyi * TH;
         for (int q = threadIdx.x; q < TH * TW; q += 256) {
@@ -1440,7 +1532,7 @@ __global__ __launch_bounds__(256) void k_degrid_lds4h(ItemSource<BATCH> src)
         __syncthreads();
 
         for (int e = it.y + wave; e < it.y + it.z; e += 4) {
-            const int4 en = entries[e];
+            const int4 en = cload(entries, e);
             const int u0 = en.x & 0xFFFF, v0 = (int)((unsigned)en.x >> 16);
             const int c0 = en.y, s = entry_width(en.w);
             const int colLo = max(u0 - Bx, 0), colHi = min(u0 + s - Bx, TW);
@@ -1478,7 +1570,7 @@ __global__ __launch_bounds__(256) void k_degrid_lds4h(ItemSource<BATCH> src)
             }
             sr = wave_sum(sr);
             si = wave_sum(si);
-            const float ps = jd.planeScale[entry_plane(en, jd)];
+            const float ps = cload(jd.planeScale, entry_plane(en, jd));
             if (lane == 0) {
                 float *op = reinterpret_cast<float *>(jd.out + en.z);
                 unsafeAtomicAdd(op, sr * ps);
