@@ -671,23 +671,68 @@ struct ItemSource {
     }
 };
 
-// Next work item for the whole workgroup without spending LDS: thread 0 takes
-// it from the queue and publishes it in the workgroup's slot; waiting for its
-// memory counters completes the store in the CU's L2 before the barrier (a
+// Next work item for the whole workgroup without spending LDS: thread 0
+// publishes the item it acquired during the previous one (`next`, taken from
+// the queue one item ahead so that the queue's latency overlaps the work) in
+// the workgroup's slot and acquires the following one.  Waiting for its memory
+// counters completes the store in the CU's L2 before the barrier (a
 // device-scope fence would also write back the L2), and the waves, all on the
 // same CU, read the slot from that L2.  The caller reaches another barrier
-// before thread 0 publishes again, so no wave can miss a value.
+// before thread 0 publishes again, so no wave can miss a value.  Every item
+// acquired is processed: the loop ends on the first one past the end.
 template <bool BATCH>
-__device__ __forceinline__ int acquire_for_block(const ItemSource<BATCH> &src)
+__device__ __forceinline__ int acquire_for_block(const ItemSource<BATCH> &src, int &next, int nItems)
 {
     if (threadIdx.x == 0) {
-        __hip_atomic_store(src.bcast + blockIdx.x, src.acquire(), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(src.bcast + blockIdx.x, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __builtin_amdgcn_s_waitcnt(0);   // every counter of this wave drained
     }
     __syncthreads();
-    return __builtin_amdgcn_readfirstlane(
+    const int item = __builtin_amdgcn_readfirstlane(
         __hip_atomic_load(src.bcast + blockIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (threadIdx.x == 0 && item < nItems) next = src.acquire();
+    return item;
+}
+
+// The first item of a workgroup (thread 0 only; see acquire_for_block).
+template <bool BATCH>
+__device__ __forceinline__ int first_item(const ItemSource<BATCH> &src)
+{
+    return threadIdx.x == 0 ? src.acquire() : 0;
+}
+
+// The TW x TH grid tile at (Bx, By) into LDS (a workgroup of 256 threads),
+// cells outside the grid as zero.  Every thread issues all its loads, two
+// cells (16 bytes) each, before its first LDS write, so that their latencies
+// overlap instead of adding up.
+template <int TW, int TH>
+__device__ __forceinline__ void load_tile(float2 *tile, const float2 *__restrict__ grid, int gSize,
+                                          int Bx, int By)
+{
+    constexpr int kPairs = TH * TW / 2;
+    constexpr int kPer = (kPairs + 255) / 256;
+    float4 v[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int q = 2 * ((int)threadIdx.x + 256 * i);   // first cell of the pair
+        const int row = By + q / TW, col = Bx + q % TW;
+        v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (q < TH * TW && row < gSize) {
+            const float2 *g = grid + (size_t)row * gSize + col;
+            if (col + 1 < gSize) {
+                __builtin_memcpy(&v[i], g, 16);   // 8-byte aligned when gSize is odd
+            } else if (col < gSize) {
+                const float2 a = *g;
+                v[i].x = a.x;
+                v[i].y = a.y;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int q = 2 * ((int)threadIdx.x + 256 * i);
+        if (q < TH * TW) *reinterpret_cast<float4 *>(tile + q) = v[i];
+    }
 }
 
 // ---- tiled gridder ------------------------------------------------------------
@@ -711,12 +756,16 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled(ItemSource<BATCH> src)
     const int nItems = src.total();
     const int q0 = wave * R;   // first row group of this wave
 
+    int next = first_item(src);
     for (;;) {
-        if (threadIdx.x == 0) s_item = src.acquire();
+        // `next` was acquired during the previous item: the queue's latency
+        // overlaps the work
+        if (threadIdx.x == 0) s_item = next;
         __syncthreads();
         const int item = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
         if (item >= nItems) break;
+        if (threadIdx.x == 0) next = src.acquire();
         JobDesc jd;
         int local;
         src.locate(item, jd, local);
@@ -774,16 +823,16 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled(ItemSource<BATCH> src)
         float2 *__restrict__ g0 = jd.grid + (size_t)(By + q0 * RPI + ly) * gSize + col;
         const int rowsLeft = gSize - (By + q0 * RPI + ly);   // rows j*RPI < rowsLeft exist
         if (it.w) {
-            // the tile is this item's alone: plain read-modify-write
+            // the tile is this item's alone: plain read-modify-write, every read
+            // issued before the first write so that their latencies overlap
+            float2 o[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+                o[j] = (colOk && j * RPI < rowsLeft) ? g0[(size_t)j * RPI * gSize] : make_float2(0.f, 0.f);
 #pragma unroll
             for (int j = 0; j < R; ++j) {
-                if (colOk && j * RPI < rowsLeft && (ar[j] != 0.f || ai[j] != 0.f)) {
-                    float2 *gp = g0 + (size_t)j * RPI * gSize;
-                    float2 o = *gp;
-                    o.x += ar[j];
-                    o.y += ai[j];
-                    *gp = o;
-                }
+                if (colOk && j * RPI < rowsLeft && (ar[j] != 0.f || ai[j] != 0.f))
+                    g0[(size_t)j * RPI * gSize] = make_float2(o[j].x + ar[j], o[j].y + ai[j]);
             }
         } else {
 #pragma unroll
@@ -817,12 +866,16 @@ __global__ __launch_bounds__(NW * 64) void k_degrid_tiled(ItemSource<BATCH> src)
     __shared__ int s_item;
     const int nItems = src.total();
 
+    int next = first_item(src);
     for (;;) {
-        if (threadIdx.x == 0) s_item = src.acquire();
+        // `next` was acquired during the previous item: the queue's latency
+        // overlaps the work
+        if (threadIdx.x == 0) s_item = next;
         __syncthreads();
         const int item = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
         if (item >= nItems) break;
+        if (threadIdx.x == 0) next = src.acquire();
         JobDesc jd;
         int local;
         src.locate(item, jd, local);
@@ -916,12 +969,16 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled2(ItemSource<BATCH> src)
     const int nItems = src.total();
     const int q0 = wave * R;
 
+    int next = first_item(src);
     for (;;) {
-        if (threadIdx.x == 0) s_item = src.acquire();
+        // `next` was acquired during the previous item: the queue's latency
+        // overlaps the work
+        if (threadIdx.x == 0) s_item = next;
         __syncthreads();
         const int item = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
         if (item >= nItems) break;
+        if (threadIdx.x == 0) next = src.acquire();
         JobDesc jd;
         int local;
         src.locate(item, jd, local);
@@ -1021,19 +1078,24 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled2(ItemSource<BATCH> src)
         float2 *__restrict__ g0 = jd.grid + (size_t)(By + q0 * RPI + ly) * gSize + col;
         const bool c0ok = col < gSize, c1ok = col + 1 < gSize;
         if (it.w) {
-            // the tile is this item's alone: plain read-modify-write
+            // the tile is this item's alone: plain read-modify-write, every read
+            // issued before the first write so that their latencies overlap
+            float2 o0[R], o1[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const float2 *gp = g0 + (size_t)j * RPI * gSize;
+                const bool rok = j * RPI < rowsLeft;
+                o0[j] = (rok && c0ok) ? gp[0] : make_float2(0.f, 0.f);
+                o1[j] = (rok && c1ok) ? gp[1] : make_float2(0.f, 0.f);
+            }
 #pragma unroll
             for (int j = 0; j < R; ++j) {
                 float2 *gp = g0 + (size_t)j * RPI * gSize;
                 const bool rok = j * RPI < rowsLeft;
-                if (rok && c0ok && (ar0[j] != 0.f || ai0[j] != 0.f)) {
-                    const float2 o = gp[0];
-                    gp[0] = make_float2(o.x + ar0[j], o.y + ai0[j]);
-                }
-                if (rok && c1ok && (ar1[j] != 0.f || ai1[j] != 0.f)) {
-                    const float2 o = gp[1];
-                    gp[1] = make_float2(o.x + ar1[j], o.y + ai1[j]);
-                }
+                if (rok && c0ok && (ar0[j] != 0.f || ai0[j] != 0.f))
+                    gp[0] = make_float2(o0[j].x + ar0[j], o0[j].y + ai0[j]);
+                if (rok && c1ok && (ar1[j] != 0.f || ai1[j] != 0.f))
+                    gp[1] = make_float2(o1[j].x + ar1[j], o1[j].y + ai1[j]);
             }
         } else {
             // Atomic flush with every instruction on contiguous floats: each row
@@ -1086,12 +1148,16 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled4h(ItemSource<BATCH> src)
     const int nItems = src.total();
     const int q0 = wave * R;
 
+    int next = first_item(src);
     for (;;) {
-        if (threadIdx.x == 0) s_item = src.acquire();
+        // `next` was acquired during the previous item: the queue's latency
+        // overlaps the work
+        if (threadIdx.x == 0) s_item = next;
         __syncthreads();
         const int item = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
         if (item >= nItems) break;
+        if (threadIdx.x == 0) next = src.acquire();
         JobDesc jd;
         int local;
         src.locate(item, jd, local);
@@ -1214,8 +1280,9 @@ __global__ __launch_bounds__(256) void k_grid_lds(ItemSource<BATCH> src)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nItems = src.total();
 
+    int next = first_item(src);
     for (;;) {
-        const int item = acquire_for_block(src);
+        const int item = acquire_for_block(src, next, nItems);
         __syncthreads();   // every wave has read the slot
         if (item >= nItems) break;
         JobDesc jd;
@@ -1303,10 +1370,11 @@ __global__ __launch_bounds__(256) void k_degrid_lds(ItemSource<BATCH> src)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nItems = src.total();
 
+    int next = first_item(src);
     for (;;) {
         // the barrier inside also means every wave is done with the previous
         // tile; the one after the tile load, that every wave has read the slot
-        const int item = acquire_for_block(src);
+        const int item = acquire_for_block(src, next, nItems);
         if (item >= nItems) break;
         JobDesc jd;
         int local;
@@ -1318,11 +1386,7 @@ __global__ __launch_bounds__(256) void k_degrid_lds(ItemSource<BATCH> src)
         const int4 it = cload(jd.items, local);
         const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
         const int Bx = txi * TW, By = tyi * TH;
-        for (int q = threadIdx.x; q < TH * TW; q += 256) {
-            const int row = By + q / TW, col = Bx + q % TW;
-            tile[q] = (row < gSize && col < gSize) ? grid[(size_t)row * gSize + col]
-                                                   : make_float2(0.f, 0.f);
-        }
+        load_tile<TW, TH>(tile, grid, gSize, Bx, By);
         __syncthreads();
 
         for (int e = it.y + wave; e < it.y + it.z; e += 4) {
@@ -1386,10 +1450,11 @@ __global__ __launch_bounds__(256) void k_degrid_lds2(ItemSource<BATCH> src)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nItems = src.total();
 
+    int next = first_item(src);
     for (;;) {
         // the barrier inside also means every wave is done with the previous
         // tile; the one after the tile load, that every wave has read the slot
-        const int item = acquire_for_block(src);
+        const int item = acquire_for_block(src, next, nItems);
         if (item >= nItems) break;
         JobDesc jd;
         int local;
@@ -1401,11 +1466,7 @@ __global__ __launch_bounds__(256) void k_degrid_lds2(ItemSource<BATCH> src)
         const int4 it = cload(jd.items, local);
         const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
         const int Bx = txi * TW, By = tyi * TH;
-        for (int q = threadIdx.x; q < TH * TW; q += 256) {
-            const int row = By + q / TW, col = Bx + q % TW;
-            tile[q] = (row < gSize && col < gSize) ? grid[(size_t)row * gSize + col]
-                                                   : make_float2(0.f, 0.f);
-        }
+        load_tile<TW, TH>(tile, grid, gSize, Bx, By);
         __syncthreads();
 
         const int eEnd = it.y + it.z;
@@ -1509,10 +1570,11 @@ __global__ __launch_bounds__(256) void k_degrid_lds4h(ItemSource<BATCH> src)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nItems = src.total();
 
+    int next = first_item(src);
     for (;;) {
         // the barrier inside also means every wave is done with the previous
         // tile; the one after the tile load, that every wave has read the slot
-        const int item = acquire_for_block(src);
+        const int item = acquire_for_block(src, next, nItems);
         if (item >= nItems) break;
         JobDesc jd;
         int local;
@@ -1524,11 +1586,7 @@ __global__ __launch_bounds__(256) void k_degrid_lds4h(ItemSource<BATCH> src)
         const int4 it = cload(jd.items, local);
         const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
         const int Bx = txi * TW, By = tyi * TH;
-        for (int q = threadIdx.x; q < TH * TW; q += 256) {
-            const int row = By + q / TW, col = Bx + q % TW;
-            tile[q] = (row < gSize && col < gSize) ? grid[(size_t)row * gSize + col]
-                                                   : make_float2(0.f, 0.f);
-        }
+        load_tile<TW, TH>(tile, grid, gSize, Bx, By);
         __syncthreads();
 
         for (int e = it.y + wave; e < it.y + it.z; e += 4) {
