@@ -401,7 +401,9 @@ tcv_status tcv_range_errors(tcv_ctx *ctx, long long *count)
 
 // Defaults (a tuning field <= 0 means auto; xcd_split < 0 means auto):
 // tile width from the kernel width, 64-row tiles, 2048-entry work items, up to
-// 8192 C-slice keys, entries ordered by key in phases of about 4 MB of taps
+// 2048 C-slice keys (the binning's (tile, key) histogram stays small enough for
+// its atomics to be cheap; finer keys ordered the entries no better for the
+// kernels), entries ordered by key in phases of about 5.5 MB of taps
 // (concurrently running workgroups share the phase's slices in L2; fewer
 // phases when work items would get short), and, with 8 or more phases, work
 // items ordered XCD-major with one queue per XCD so each XCD walks its own
@@ -442,7 +444,7 @@ static void auto_tuning(int maxS, const tcv_geometry *g, tcv_tuning *t)
     int minS = maxS;
     for (int k = 0; k < g->nPlanes; k++) minS = std::min(minS, g->sSize[k]);
     const long long slices = std::max<long long>(1, g->cSize / ((long long)minS * minS));
-    if (t->slice_keys <= 0) t->slice_keys = (int)std::min<long long>(8192, slices);
+    if (t->slice_keys <= 0) t->slice_keys = (int)std::min<long long>(2048, slices);
     if (t->phases <= 0)
         t->phases = default_phases(g->cSize, t->slice_keys, g, maxS, t->tile_w, t->tile_h,
                                    kDegridPhaseBytes);
