@@ -198,6 +198,32 @@ tcv_status tcv_copy(tcv_ctx *ctx, void *d_dst, const void *d_src, size_t bytes)
     return launch_copy(ctx, d_dst, d_src, bytes / 16);
 }
 
+tcv_status tcv_read_bandwidth(tcv_ctx *ctx, const void *d_buf, size_t bytes, int rounds,
+                              double *gbps)
+{
+    if (!ctx) return TCV_ERR_INVALID;
+    if (!d_buf || !gbps || rounds < 8 || (rounds % 8) || bytes < 4096 || (bytes & (bytes - 1)) ||
+        bytes > (64ull << 30) || ((uintptr_t)d_buf % 16))
+        return tcv_fail(ctx, TCV_ERR_INVALID,
+                        "tcv_read_bandwidth: 16-byte aligned power-of-two buffer of 4 KiB..64 GiB, "
+                        "rounds a positive multiple of 8");
+    if (!ctx->d_bcast) return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_read_bandwidth: context not set up");
+    unsigned *sink = reinterpret_cast<unsigned *>(ctx->d_bcast);   // written only by chance
+    tcv_status st = launch_read_stream(ctx, d_buf, bytes / 16, rounds, sink);   // warm-up
+    if (st != TCV_OK) return st;
+    const int reps = 5;
+    TCV_HIP(ctx, hipEventRecord(ctx->timer0, ctx->stream));
+    for (int r = 0; r < reps && st == TCV_OK; ++r) st = launch_read_stream(ctx, d_buf, bytes / 16, rounds, sink);
+    if (st != TCV_OK) return st;
+    TCV_HIP(ctx, hipEventRecord(ctx->timer1, ctx->stream));
+    TCV_HIP(ctx, hipEventSynchronize(ctx->timer1));
+    float ms = 0.f;
+    TCV_HIP(ctx, hipEventElapsedTime(&ms, ctx->timer0, ctx->timer1));
+    const double window = (double)std::max(1, ctx->cus) * 8.0 * 256.0 * 16.0;   // bytes per round
+    *gbps = reps * window * rounds / (ms * 1e-3) / 1e9;
+    return TCV_OK;
+}
+
 tcv_status tcv_timer_start(tcv_ctx *ctx)
 {
     if (!ctx) return TCV_ERR_INVALID;

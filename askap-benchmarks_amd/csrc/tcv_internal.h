@@ -72,6 +72,7 @@ struct GeomDev {
 // Launchers implemented in tcv_kernels.hip.  All enqueue on ctx->stream.
 tcv_status launch_fill_u32(tcv_ctx *ctx, void *p, unsigned value, size_t n32);
 tcv_status launch_copy(tcv_ctx *ctx, void *dst, const void *src, size_t n16);
+tcv_status launch_read_stream(tcv_ctx *ctx, const void *buf, size_t n16, int rounds, unsigned *sink);
 tcv_status launch_offsets(tcv_ctx *ctx, const double *u, const double *v,
                           const double *w, int nSamples, const double *d_freq,
                           int nChan, double cellSize, double wCellSize, int wSize,

@@ -105,6 +105,29 @@ __global__ __launch_bounds__(256) void k_copy_u4(uint4 *__restrict__ dst,
     for (; i < n16; i += stride) dst[i] = src[i];
 }
 
+// Streaming read of a power-of-two buffer, 16 B per lane (the read-bandwidth
+// probe): each round the grid reads one window of its own size (thread g reads
+// element base + g, coalesced), and the window moves on by its size plus an odd
+// offset, so no CU meets data its L1 still holds.  A buffer that fits the L2s
+// measures their bandwidth; a large one, HBM's.  The sum only keeps the loads.
+__global__ __launch_bounds__(256) void k_read_stream(const uint4 *__restrict__ p, unsigned mask,
+                                                     unsigned adv, int rounds, unsigned *sink)
+{
+    const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned base = 0, acc = 0;
+    for (int r = 0; r < rounds; r += 8) {
+        uint4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            v[u] = p[(base + g) & mask];
+            base += adv;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+    }
+    if (acc == 0x9e3779b9u) sink[0] = acc;
+}
+
 // ---- a6: FP64 index set-up ----------------------------------------------------
 // attic/tConvolveMPI/Benchmark.cc:293-322 and current/tConvolveMPI/Benchmark.cc:610-643.
 // Same expression order as the reference; contraction off so every product and
@@ -1758,6 +1781,16 @@ tcv_status launch_copy(tcv_ctx *ctx, void *dst, const void *src, size_t n16)
     const size_t blocks = std::min<size_t>((n16 + 255) / 256, (size_t)ctx->cus * 16);
     hipLaunchKernelGGL(k_copy_u4, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
                        (uint4 *)dst, (const uint4 *)src, n16);
+    TCV_HIP(ctx, hipGetLastError());
+    return TCV_OK;
+}
+
+tcv_status launch_read_stream(tcv_ctx *ctx, const void *buf, size_t n16, int rounds, unsigned *sink)
+{
+    const unsigned blocks = (unsigned)std::max(1, ctx->cus) * 8u;
+    const unsigned window = blocks * 256u;
+    hipLaunchKernelGGL(k_read_stream, dim3(blocks), dim3(256), 0, ctx->stream, (const uint4 *)buf,
+                       (unsigned)(n16 - 1), window + 40961u, rounds, sink);
     TCV_HIP(ctx, hipGetLastError());
     return TCV_OK;
 }

@@ -105,6 +105,7 @@ SIGS = {
     "tcv_attic_initC_device": (I, [P, D, D, I, I, I, PI, PD, P]),
     "tcv_mem_info": (I, [P, ct.POINTER(ct.c_size_t), ct.POINTER(ct.c_size_t)]),
     "tcv_copy": (I, [P, P, P, ct.c_size_t]),
+    "tcv_read_bandwidth": (I, [P, P, ct.c_size_t, ct.c_int, ct.POINTER(ct.c_double)]),
     "tcv_grid_psf": (I, [P, P, P, P]),
     "tcv_fft_exec": (I, [P, P, I, I]),
     "tcv_fft_fix": (I, [P, P, LL, ct.c_float]),

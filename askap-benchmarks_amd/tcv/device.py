@@ -104,6 +104,14 @@ class Context:
         self.check(lib().tcv_mem_info(self.h, ct.byref(f), ct.byref(t)), "mem_info")
         return f.value, t.value
 
+    def read_bandwidth(self, buf: "DeviceArray", rounds: int = 256) -> float:
+        """Streaming read bandwidth (GB/s) over `buf` (power-of-two bytes): the
+        L2's when it fits there, HBM's for a large buffer (tcv_read_bandwidth)."""
+        g = ct.c_double()
+        self.check(lib().tcv_read_bandwidth(self.h, buf.ptr, buf.nbytes, int(rounds), ct.byref(g)),
+                   "read_bandwidth")
+        return g.value
+
     def range_errors(self) -> int:
         c = ct.c_longlong()
         self.check(lib().tcv_range_errors(self.h, ct.byref(c)), "range_errors")

@@ -47,10 +47,12 @@ TUNING_KEYS = ("tile_w", "tile_h", "chunk", "slice_keys", "phases", "xcd_split",
                "degrid_kernel", "grid_phases")
 
 
-def measured_traffic(kernel, info):
-    """HBM bytes per launch of `kernel` from the committed counter summary
-    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel
-    calibration), when it was measured with the same tuning; else None."""
+def measured_traffic(kernel, info, field="bytes_per_launch"):
+    """Bytes per launch of `kernel` from the committed counter summary, when it
+    was measured with the same tuning; else None.  bytes_per_launch: between
+    L2 and the memory fabric (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration); l2_read_bytes_per_launch: from L2 into the L1s
+    (TCP_TCC_READ_REQ times the request size the copy kernel shows)."""
     try:
         lines = open(TRAFFIC_FILE).read().splitlines()
     except OSError:
@@ -61,7 +63,20 @@ def measured_traffic(kernel, info):
             rec = json.loads(ln.split(":", 1)[1])
     if not rec or any(rec.get("tuning", {}).get(k) != info.get(k) for k in TUNING_KEYS):
         return None
-    return rec.get("bytes_per_launch", {}).get(kernel)
+    return rec.get(field, {}).get(kernel)
+
+
+def read_ceilings(ctx):
+    """Streaming read bandwidth (GB/s) of the device's levels, measured live:
+    a 2 MiB buffer (held by the L2s), 64 MiB (Infinity Cache) and 1 GiB (HBM)."""
+    import numpy as np
+    out = {}
+    for name, nbytes in (("l2", 2 << 20), ("infinity_cache", 64 << 20), ("hbm", 1 << 30)):
+        buf = ctx.empty(nbytes // 4, np.uint32)
+        buf.zero()
+        out[name] = ctx.read_bandwidth(buf)
+        buf.free()
+    return out
 
 
 def calibration_copy(ctx, nbytes=1 << 30, reps=2):
@@ -322,6 +337,8 @@ def main():
         job.degrid("tiled")
 
     calibration_copy(ctx)
+    ceilings = read_ceilings(ctx)
+    log("read ceilings GB/s: " + ", ".join(f"{k} {v:.0f}" for k, v in ceilings.items()))
     for _ in range(args.warmup):
         step()
     ctx.sync()
@@ -357,7 +374,13 @@ def main():
                 "kernel": dom,
                 "avg_launch_ms": avg_ms, "alg_bytes_per_launch": bytes_l,
                 "measured_hbm_GBps": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
-                "reference_model_GBps": ref_model / (avg_ms * 1e-3) / 1e9}
+                "reference_model_GBps": ref_model / (avg_ms * 1e-3) / 1e9,
+                "read_ceilings_GBps": ceilings}
+    l2_bytes = measured_traffic(dom, info, "l2_read_bytes_per_launch")
+    if l2_bytes:
+        roofline["l2_read_bytes_per_launch"] = l2_bytes
+        roofline["l2_read_GBps"] = l2_bytes / (avg_ms * 1e-3) / 1e9
+        roofline["l2_read_frac_of_l2_ceiling"] = roofline["l2_read_GBps"] / ceilings["l2"]
     per_kernel = {k: {"launches": v[0], "avg_ms": (v[1] / v[0]) if v[0] else None}
                   for k, v in stats.items()}
 

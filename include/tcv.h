@@ -69,6 +69,15 @@ tcv_status tcv_zero(tcv_ctx *ctx, void *d_ptr, size_t bytes);
  * hardware memory-traffic counters against the gridding kernels. */
 tcv_status tcv_copy(tcv_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
 
+/* Streaming read bandwidth of the device, in GB/s (blocking): every CU reads a
+ * power-of-two buffer of `bytes` (4 KiB..64 GiB, 16-byte aligned) in 16-byte
+ * loads, `rounds` (a multiple of 8) windows of 8 workgroups per CU each, timed
+ * over five launches after a warm-up.  No counterpart in the reference: a
+ * buffer that fits the L2s measures the L2 read bandwidth, a large one HBM's,
+ * the ceilings beside which the gridding kernels' traffic is reported. */
+tcv_status tcv_read_bandwidth(tcv_ctx *ctx, const void *d_buf, size_t bytes, int rounds,
+                              double *gbps);
+
 /* ---- timing (hipEvents on the context stream) ----------------------------- */
 
 tcv_status tcv_timer_start(tcv_ctx *ctx);

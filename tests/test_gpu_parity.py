@@ -317,6 +317,19 @@ def test_copy_kernel_bit_exact(ctx):
         assert np.array_equal(dst.download(), a)
 
 
+def test_read_bandwidth_probe(ctx):
+    """The streaming-read probe returns a positive rate for a power-of-two
+    buffer and refuses other sizes before launching anything."""
+    buf = ctx.empty((4 << 20) // 4, np.uint32)
+    buf.zero()
+    g = ctx.read_bandwidth(buf, rounds=64)
+    assert 100.0 < g < 1e6
+    with pytest.raises(tcv.TcvError):
+        ctx.read_bandwidth(ctx.empty(3000, np.uint32))
+    with pytest.raises(tcv.TcvError):
+        ctx.read_bandwidth(buf, rounds=12)
+
+
 def exact_problem(seed, nVis, gSize, Sz, cluster):
     """Inputs whose every partial sum is exact in FP32: C taps are multiples
     of 1/4 in [-1, 1], data and grid values small integers.  Results then do
