@@ -59,6 +59,14 @@ __device__ __forceinline__ unsigned umad24(int a, int b, unsigned c)
     return r;
 }
 
+// a * b + c (24-bit signed a, b; b wave-uniform)
+__device__ __forceinline__ unsigned imad24(int a, int b, unsigned c)
+{
+    unsigned r;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+    return r;
+}
+
 __device__ __forceinline__ float2 load_c(__amdgpu_buffer_rsrc_t r, unsigned voff)
 {
     return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
@@ -1459,10 +1467,12 @@ __global__ __launch_bounds__(256) void k_degrid_lds(ItemSource<BATCH> src)
 
 // Degridder with the LDS tile and tap pairs: lanes walk each entry's
 // footprint on the tile densely, one pair of adjacent cells per lane and step
-// (a 16-byte C load, two LDS cells); with an odd width the last column follows
-// as one pass with a lane per row.  No pair leaves the footprint, so every tap
-// read is one the reference reads.  Index arithmetic is 24-bit (full-rate),
-// and only the last step of an entry checks for lanes past its end.
+// (a 16-byte C load, two LDS cells).  An odd width takes one more column whose
+// cells count as zero (the taps it reads are neighbours in the table: an
+// exact zero contribution for a finite table); where such a tap could lie
+// outside the table, the last column gets one pass with a lane per row
+// instead.  Index arithmetic is 24-bit (full-rate), and only the last step of
+// an entry checks for lanes past its end.
 template <int TW, int TH, bool HALF, bool BATCH>
 __global__ __launch_bounds__(256) void k_degrid_lds2(ItemSource<BATCH> src)
 {
@@ -1483,6 +1493,7 @@ __global__ __launch_bounds__(256) void k_degrid_lds2(ItemSource<BATCH> src)
         int local;
         src.locate(item, jd, local);
         const __amdgpu_buffer_rsrc_t rc = make_rsrc(jd.C, jd.cBytes);
+        const int nTaps = (int)(jd.cBytes / TB);
         const int4 *__restrict__ entries = jd.entries;
         const float2 *__restrict__ grid = jd.grid;
         const int gSize = jd.gSize;
@@ -1501,21 +1512,40 @@ __global__ __launch_bounds__(256) void k_degrid_lds2(ItemSource<BATCH> src)
             const int rowLo = max(v0 - By, 0), rowHi = min(v0 + s - By, TH);
             const int w = colHi - colLo, h = rowHi - rowLo;
             if (w <= 0 || h <= 0) continue;
-            const int wp = w >> 1;                            // whole pairs per row
+            // An odd width is walked as pairs over one more column, whose cells take
+            // a zero grid value: the next column when the footprint ends inside the
+            // tile (mode 1; its tap is the next one in the table), else the one
+            // before (mode 2; the tap before).  Where that tap could lie outside the
+            // table, the last column gets a pass of its own (mode 0).
+            int mode = 0, colA = colLo, wA = w;
+            if (w & 1) {
+                if (colHi < TW && c0 + s * s < nTaps) {
+                    mode = 1;
+                    wA = w + 1;
+                } else if (colLo > 0 && c0 > 0) {
+                    mode = 2;
+                    colA = colLo - 1;
+                    wA = w + 1;
+                }
+            }
+            const int maskCol = mode == 1 ? colA + wA - 2 : colA;   // pair's first column
+            const int wp = wA >> 1;                                 // pairs per row
             const int n = wp * h;
             const float rw = 1.0f / (float)max(wp, 1);
-            // byte offsets of cell (rowLo, colLo): in C (its tap) and in the tile
-            const unsigned cB = (unsigned)(c0 + (By + rowLo - v0) * s + Bx + colLo - u0) * TB;
-            const unsigned lB = (unsigned)(rowLo * TW + colLo) * 8u;
+            // byte offsets of cell (rowLo, colA): in C (its tap) and in the tile
+            const unsigned cB = (unsigned)(c0 + (By + rowLo - v0) * s + Bx + colA - u0) * TB;
+            const unsigned lB = (unsigned)(rowLo * TW + colA) * 8u;
             const int sB = s * (int)TB;
             float sr = 0.f, si = 0.f;
             // one step: pair k = p - rl * wp of row rl for p = p0 + lane + 64 u.
             // The offsets of the pair, base + rl * sB + 2 k TB (C) and
             // base + rl * TW * 8 + 16 k (tile), are written as
             // (base + 2 p TB) + rl * (sB - 2 wp TB): one 24-bit multiply-add each.
-            const int aC = sB - 2 * (int)TB * wp, aL = TW * 8 - 16 * wp;   // both >= 0
-            auto step = [&](int p0, auto tailTag) {
+            // aL >= 0; aC < 0 when a kernel narrower than the tile gets its added column
+            const int aC = sB - 2 * (int)TB * wp, aL = TW * 8 - 16 * wp;
+            auto step = [&](int p0, auto tailTag, auto modeTag) {
                 constexpr bool TAIL = decltype(tailTag)::value;
+                constexpr int MODE = decltype(modeTag)::value;
                 float4 c[kLdsU];
                 float2 g0[kLdsU], g1[kLdsU];
                 const int pl = p0 + lane;
@@ -1526,7 +1556,7 @@ __global__ __launch_bounds__(256) void k_degrid_lds2(ItemSource<BATCH> src)
                     // row = floor((p + 0.5) / wp): p + 0.5 is at least 1/(2 wp) from a
                     // multiple of wp, far beyond the rounding of the float arithmetic
                     const int rl = (int)fmaf(pf, rw, (float)(64 * u) * rw);
-                    unsigned co = umad24(rl, aC, cP) + (unsigned)(64 * u) * (2u * TB);
+                    unsigned co = imad24(rl, aC, cP) + (unsigned)(64 * u) * (2u * TB);
                     unsigned lo = umad24(rl, aL, lP) + (unsigned)(64 * u) * 16u;
                     bool ok = true;
                     if constexpr (TAIL) {
@@ -1542,6 +1572,12 @@ __global__ __launch_bounds__(256) void k_degrid_lds2(ItemSource<BATCH> src)
                     if constexpr (TAIL) {
                         if (!ok) g0[u] = g1[u] = make_float2(0.f, 0.f);
                     }
+                    if constexpr (MODE != 0) {
+                        // the pair on the added column: its outside cell takes zero
+                        const bool edge = (int)((lo >> 3) & (TW - 1)) == maskCol;
+                        if (MODE == 1 && edge) g1[u] = make_float2(0.f, 0.f);
+                        if (MODE == 2 && edge) g0[u] = make_float2(0.f, 0.f);
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < kLdsU; ++u) {
@@ -1551,10 +1587,15 @@ __global__ __launch_bounds__(256) void k_degrid_lds2(ItemSource<BATCH> src)
                     si = fmaf(g1[u].x, c[u].w, fmaf(g1[u].y, c[u].z, si));
                 }
             };
-            int p0 = 0;
-            for (; p0 + 64 * kLdsU <= n; p0 += 64 * kLdsU) step(p0, std::false_type{});
-            if (p0 < n) step(p0, std::true_type{});
-            if (w & 1) {
+            auto walk = [&](auto modeTag) {
+                int p0 = 0;
+                for (; p0 + 64 * kLdsU <= n; p0 += 64 * kLdsU) step(p0, std::false_type{}, modeTag);
+                if (p0 < n) step(p0, std::true_type{}, modeTag);
+            };
+            if (mode == 1) walk(std::integral_constant<int, 1>{});
+            else if (mode == 2) walk(std::integral_constant<int, 2>{});
+            else walk(std::integral_constant<int, 0>{});
+            if ((w & 1) && mode == 0) {
                 // the last column, a lane per row
                 const bool ok = lane < h;
                 const unsigned col = (unsigned)(w - 1);
