@@ -387,6 +387,35 @@ def test_pair_kernels_every_width_exact(ctx, Sz):
         assert np.array_equal(gpu_degrid(ctx, vis, G, C, "tiled", tuning), oref), (tw, th)
 
 
+@pytest.mark.parametrize("Sz", [3, 9, 33, 129])
+def test_pair_kernels_first_and_last_slice_exact(ctx, Sz):
+    """Footprints whose C slice is the table's first or last, at every offset
+    around a tile edge: the pair kernels then load the pair that starts or ends
+    on the edge tap (no tap outside the table is read); bit-exact."""
+    gSize = 256
+    iu, iv, co, data, C, G = exact_problem(500 + Sz, 400, gSize, Sz, cluster=None)
+    half = Sz // 2
+    n_slices = C.size // (Sz * Sz)
+    extra_u = np.clip(np.array([half, gSize - half - 1] + [64 + d for d in range(-Sz - 1, 3)] +
+                               [128 + d for d in range(-Sz - 1, 3)], np.int32), half, gSize - half - 1)
+    extra_v = np.clip(np.arange(extra_u.size, dtype=np.int32) * 11 % (gSize - Sz + 1), 0, gSize - Sz)
+    iu = np.concatenate([iu, extra_u, extra_u])
+    iv = np.concatenate([iv, extra_v, (gSize - Sz - extra_v)]).astype(np.int32)
+    first_last = np.array([0, (n_slices - 1) * Sz * Sz], np.int32)
+    co = np.concatenate([co, first_last[np.arange(2 * extra_u.size) % 2]]).astype(np.int32)
+    co[: co.size // 4] = first_last[np.arange(co.size // 4) % 2]
+    rng = np.random.default_rng(Sz + 7)
+    data = np.concatenate([data, (rng.integers(-3, 4, 2 * extra_u.size) +
+                                  1j * rng.integers(-3, 4, 2 * extra_u.size)).astype(np.complex64)])
+    vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
+    gref = O.grid(data, iu, iv, co, C, gSize, Sz)
+    oref = O.degrid(G, iu, iv, co, C, Sz)
+    for tw, th in ((64, 64), (32, 64), (16, 32)):
+        tuning = dict(tile_w=tw, tile_h=th, kernel=3, degrid_kernel=3, phases=2, slice_keys=4, chunk=48)
+        assert np.array_equal(gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning), gref), (tw, th)
+        assert np.array_equal(gpu_degrid(ctx, vis, G, C, "tiled", tuning), oref), (tw, th)
+
+
 def test_tuning_kernel_field_validated(ctx):
     Sz, gSize = 33, 128
     iu, iv, co, data, C = small_problem(78, 50, gSize, Sz)
