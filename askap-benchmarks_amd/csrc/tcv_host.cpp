@@ -555,6 +555,7 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
     size_t n = 0;
     n += align256(sizeof(int) * g->nPlanes);
     n += align256(sizeof(int4) * sh.maxEntries);
+    if (plan_two_level(sh)) n += align256(sizeof(int4) * sh.maxEntries);
     n += align256(sizeof(int4) * p->maxItems);
     n += 3 * align256(sizeof(int) * (sh.nKeys + 1));
     n += 2 * align256(sizeof(int) * (slots + 1));
@@ -581,6 +582,7 @@ void plan_bind(tcv_plan *p, char *block)
     };
     p->d_sSize = (int *)take(sizeof(int) * p->nPlanes);
     p->entries = (int4 *)take(sizeof(int4) * sh.maxEntries);
+    p->entriesTmp = plan_two_level(sh) ? (int4 *)take(sizeof(int4) * sh.maxEntries) : nullptr;
     p->items = (int4 *)take(sizeof(int4) * p->maxItems);
     p->counts = (int *)take(sizeof(int) * (sh.nKeys + 1));
     p->starts = (int *)take(sizeof(int) * (sh.nKeys + 1));

@@ -132,6 +132,7 @@ struct tcv_plan {
     std::vector<int> sSizeHost;
     int *d_sSize = nullptr;
     int4 *entries = nullptr;      // maxEntries
+    int4 *entriesTmp = nullptr;   // maxEntries: tile-grouped entries (two-level binning), or null
     int4 *items = nullptr;        // maxItems
     long long maxItems = 0;
     int *counts = nullptr;        // nKeys + 1
@@ -160,6 +161,10 @@ struct tcv_plan {
 tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tuning, tcv_plan *p,
                       size_t *bytes);
 void plan_bind(tcv_plan *p, char *block);
+// Whether the plan bins in two levels (by tile in LDS, then by key per tile):
+// the tile histogram must fit the LDS binning kernel and the per-tile key
+// histogram one workgroup's LDS.
+bool plan_two_level(const PlanShape &sh);
 
 struct tcv_batch {
     int nJobs = 0;

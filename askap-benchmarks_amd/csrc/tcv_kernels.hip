@@ -499,6 +499,47 @@ __global__ __launch_bounds__(256) void k_plan_bin_lds(int pass, GeomDev g, PlanD
     }
 }
 
+// Second level of the two-level binning: one workgroup per tile orders the
+// tile's entries (grouped by tile by the first level, in `tmp`) by C-slice
+// key with an LDS counting sort, writes them to `entries` and the tile's key
+// offsets to starts[tile * k2 + key].  Block 0 also writes the total at
+// starts[nTiles * k2].
+__global__ __launch_bounds__(256) void k_plan_tile_keys(PlanDevArgs a, const int *__restrict__ tileStart,
+                                                        const int4 *__restrict__ tmp,
+                                                        int4 *__restrict__ entries,
+                                                        int *__restrict__ starts)
+{
+    extern __shared__ int lds[];
+    int *hist = lds, *cur = lds + a.k2;
+    const int t = blockIdx.x;
+    const int b = tileStart[t], e = tileStart[t + 1];
+    for (int k = threadIdx.x; k < a.k2; k += blockDim.x) hist[k] = 0;
+    __syncthreads();
+    for (int i = b + threadIdx.x; i < e; i += blockDim.x)
+        atomicAdd(&hist[(int)(((long long)tmp[i].y * a.k2) / a.cSize)], 1);
+    __syncthreads();
+    // exclusive scan of hist: each thread a run of consecutive keys
+    const int per = (a.k2 + blockDim.x - 1) / blockDim.x;
+    const int k0 = min((int)threadIdx.x * per, a.k2), k1 = min(k0 + per, a.k2);
+    int run = 0;
+    for (int k = k0; k < k1; ++k) run += hist[k];
+    int total;
+    int off = block_exclusive_scan(run, &total);
+    for (int k = k0; k < k1; ++k) {
+        const int c = hist[k];
+        cur[k] = off;
+        starts[(long long)t * a.k2 + k] = b + off;
+        off += c;
+    }
+    if (t == 0 && threadIdx.x == 0) starts[(long long)a.nTiles * a.k2] = tileStart[a.nTiles];
+    __syncthreads();
+    for (int i = b + threadIdx.x; i < e; i += blockDim.x) {
+        const int4 ent = tmp[i];
+        const int key = (int)(((long long)ent.y * a.k2) / a.cSize);
+        entries[b + atomicAdd(&cur[key], 1)] = ent;
+    }
+}
+
 // Work items: each (phase, tile) range of entries is cut into chunks of at most
 // `chunk` entries.  Items are ordered phase-major so concurrently running
 // workgroups read neighbouring C slices.  Item = {tile, first, count, exclusive}.
@@ -1940,12 +1981,40 @@ tcv_status plan_build_kernels(tcv_ctx *ctx, tcv_plan *p, const GeomDev &g)
         TCV_HIP(ctx, hipGetLastError());
         return TCV_OK;
     };
-    if ((st = bin(0)) != TCV_OK) return st;
-    if ((st = scan_exclusive(ctx, p->counts, p->starts, sh.nKeys, p->scanTmp)) != TCV_OK)
-        return st;
-    TCV_HIP(ctx, hipMemcpyAsync(p->cursor, p->starts, sizeof(int) * sh.nKeys,
-                                hipMemcpyDeviceToDevice, ctx->stream));
-    if ((st = bin(1)) != TCV_OK) return st;
+    if (p->entriesTmp) {
+        // Two levels: bin by tile alone (the tile histogram fits LDS, so each
+        // block touches each global tile counter at most once), then order
+        // every tile's entries by C-slice key in its own workgroup.  Global
+        // atomics drop from one per entry to one per (block, tile).
+        PlanDevArgs t1 = a;
+        t1.k2 = 1;
+        int *tileCount = p->counts, *tileStart = p->cursor, *tileCursor = p->cursor + sh.nTiles + 1;
+        const int nb1 = (g.nVis + 256 * kBinVPT - 1) / (256 * kBinVPT);
+        if (g.nVis > 0) {
+            hipLaunchKernelGGL(k_plan_bin_lds, dim3(nb1), dim3(256), (size_t)sh.nTiles * sizeof(int),
+                               ctx->stream, 0, g, t1, tileCount, tileCursor, p->entriesTmp, p->meta);
+            TCV_HIP(ctx, hipGetLastError());
+        }
+        if ((st = scan_exclusive(ctx, tileCount, tileStart, sh.nTiles, p->scanTmp)) != TCV_OK) return st;
+        TCV_HIP(ctx, hipMemcpyAsync(tileCursor, tileStart, sizeof(int) * sh.nTiles,
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+        if (g.nVis > 0) {
+            hipLaunchKernelGGL(k_plan_bin_lds, dim3(nb1), dim3(256), (size_t)2 * sh.nTiles * sizeof(int),
+                               ctx->stream, 1, g, t1, tileCount, tileCursor, p->entriesTmp, p->meta);
+            TCV_HIP(ctx, hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_plan_tile_keys, dim3(sh.nTiles), dim3(256), (size_t)2 * sh.k2 * sizeof(int),
+                           ctx->stream, a, (const int *)tileStart, (const int4 *)p->entriesTmp,
+                           p->entries, p->starts);
+        TCV_HIP(ctx, hipGetLastError());
+    } else {
+        if ((st = bin(0)) != TCV_OK) return st;
+        if ((st = scan_exclusive(ctx, p->counts, p->starts, sh.nKeys, p->scanTmp)) != TCV_OK)
+            return st;
+        TCV_HIP(ctx, hipMemcpyAsync(p->cursor, p->starts, sizeof(int) * sh.nKeys,
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+        if ((st = bin(1)) != TCV_OK) return st;
+    }
     // work items: cuts of the sorted entries into (phase, tile) ranges
     auto items = [&](int phases, int xcd, int *itemCount, int *itemStart, int4 *itemsOut,
                      int *meta) -> tcv_status {
@@ -2069,6 +2138,11 @@ tcv_status degrid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_k
 
 #define TCV_TILE_CONFIGS(X) \
     X(64, 64) X(64, 32) X(32, 64) X(32, 32) X(16, 64) X(16, 32) X(8, 64) X(8, 32)
+
+bool plan_two_level(const PlanShape &sh)
+{
+    return sh.k2 >= 2 && sh.k2 <= 8192 && sh.nTiles <= kLdsKeys;
+}
 
 bool grid2_supported(int tw, int th)
 {
