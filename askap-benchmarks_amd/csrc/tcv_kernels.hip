@@ -513,10 +513,13 @@ __global__ __launch_bounds__(256) void k_plan_tile_keys(PlanDevArgs a, const int
     int *hist = lds, *cur = lds + a.k2;
     const int t = blockIdx.x;
     const int b = tileStart[t], e = tileStart[t + 1];
+    // key = floor(cOffset * k2 / cSize) up to rounding: monotone in cOffset and
+    // computed the same way in both passes (an FP64 product, no division)
+    const double scale = (double)a.k2 / (double)a.cSize;
+    auto key_of = [&](int c0) { return min((int)((double)c0 * scale), a.k2 - 1); };
     for (int k = threadIdx.x; k < a.k2; k += blockDim.x) hist[k] = 0;
     __syncthreads();
-    for (int i = b + threadIdx.x; i < e; i += blockDim.x)
-        atomicAdd(&hist[(int)(((long long)tmp[i].y * a.k2) / a.cSize)], 1);
+    for (int i = b + threadIdx.x; i < e; i += blockDim.x) atomicAdd(&hist[key_of(tmp[i].y)], 1);
     __syncthreads();
     // exclusive scan of hist: each thread a run of consecutive keys
     const int per = (a.k2 + blockDim.x - 1) / blockDim.x;
@@ -535,8 +538,7 @@ __global__ __launch_bounds__(256) void k_plan_tile_keys(PlanDevArgs a, const int
     __syncthreads();
     for (int i = b + threadIdx.x; i < e; i += blockDim.x) {
         const int4 ent = tmp[i];
-        const int key = (int)(((long long)ent.y * a.k2) / a.cSize);
-        entries[b + atomicAdd(&cur[key], 1)] = ent;
+        entries[b + atomicAdd(&cur[key_of(ent.y)], 1)] = ent;
     }
 }
 
