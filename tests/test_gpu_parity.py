@@ -416,6 +416,24 @@ def test_pair_kernels_first_and_last_slice_exact(ctx, Sz):
         assert np.array_equal(gpu_degrid(ctx, vis, G, C, "tiled", tuning), oref), (tw, th)
 
 
+@pytest.mark.parametrize("tw,th,keys", [(16, 32, 4), (64, 64, 1), (64, 64, 16)])
+def test_plan_binning_paths_exact(ctx, tw, th, keys):
+    """The three plan binnings give the same (bit-exact) results: (tile, key)
+    with global atomics when there are more tiles than the LDS histogram holds
+    (2048^2 grid in 16x32 tiles: 8192 tiles), (tile) only in LDS when there is
+    one key, and the two-level binning (tiles in LDS, then keys per tile)."""
+    Sz, gSize = 33, 2048
+    iu, iv, co, data, C, G = exact_problem(900 + tw + keys, 3000, gSize, Sz, cluster=None)
+    vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
+    gref = O.grid(data, iu, iv, co, C, gSize, Sz)
+    oref = O.degrid(G, iu, iv, co, C, Sz)
+    tuning = dict(tile_w=tw, tile_h=th, slice_keys=keys, phases=min(keys, 4), chunk=64)
+    info = tcv.Plan(ctx, vis, **tuning).info()
+    assert info["entries"] > iu.size
+    assert np.array_equal(gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning), gref)
+    assert np.array_equal(gpu_degrid(ctx, vis, G, C, "tiled", tuning), oref)
+
+
 def test_tuning_kernel_field_validated(ctx):
     Sz, gSize = 33, 128
     iu, iv, co, data, C = small_problem(78, 50, gSize, Sz)
