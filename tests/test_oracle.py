@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import hashlib
 import math
+import os
 
 import numpy as np
 import pytest
@@ -274,3 +275,33 @@ def test_cpu_baseline_runner():
     iu, iv, co, data, C = small_problem(7, 400, gSize, S)
     tg, td = O.cpu_baseline(2, 300, data, iu, iv, co, C, gSize, S)
     assert tg > 0 and td > 0
+
+
+REF_TIMER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                         "ref_time_attic")
+
+
+@pytest.mark.skipif(not os.access(REF_TIMER, os.X_OK), reason="reference timing driver not built")
+def test_reference_timing_driver_computes_the_oracle(tmp_path):
+    """The CPU-baseline driver runs the reference's own kernels: per thread, the
+    degridded value of its first visibility from its own grid equals the
+    oracle's, summed over the threads (printed as `check`)."""
+    import json
+    import subprocess
+    S, gSize, n, threads, per = 17, 256, 600, 3, 150
+    iu, iv, co, data, C = small_problem(41, n, gSize, S)
+    for name, a in (("iu.i32", iu.astype(np.int32)), ("iv.i32", iv.astype(np.int32)),
+                    ("cOffset.i32", co.astype(np.int32)), ("data.c64", data.astype(np.complex64)),
+                    ("C.c64", C.astype(np.complex64))):
+        a.tofile(tmp_path / name)
+    (tmp_path / "params.txt").write_text(f"{gSize} {S // 2}\n")
+    out = subprocess.run([REF_TIMER, str(tmp_path), str(threads), str(per)], capture_output=True,
+                         text=True, check=True, timeout=120)
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    want = 0.0
+    for t in range(threads):
+        sl = slice(t * per, (t + 1) * per)
+        g = O.grid(data[sl], iu[sl], iv[sl], co[sl], C, gSize, S)
+        want += abs(O.degrid(g, iu[sl], iv[sl], co[sl], C, S)[0])
+    assert r["check"] == pytest.approx(want, rel=1e-5)
+    assert r["grid_s"] > 0 and r["degrid_s"] > 0
