@@ -104,6 +104,10 @@ struct JobDesc {
     int planeTaps;              // FP16: > 0 => plane = cOffset / planeTaps
     float dataScale;            // FP16: visibility scale
     int nVis;
+    // work items whose first entry's cOffset is below cEdgeLo, or whose last
+    // entry's is above cEdgeHi, may hold entries on the table's first or last
+    // slice (entries are ordered by C-slice key within an item)
+    int cEdgeLo, cEdgeHi;
 };
 
 constexpr int kMaxPersistentBlocks = 8192;

@@ -1057,14 +1057,21 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled2(ItemSource<BATCH> src)
         int local;
         src.locate(item, jd, local);
         const __amdgpu_buffer_rsrc_t rc = make_rsrc(jd.C, jd.cBytes);
-        const int nTaps = (int)(jd.cBytes / TB);
         const int4 *__restrict__ entries = jd.entries;
         const int gSize = jd.gSize;
         const int4 it = cload(jd.items, local);
         const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
         const int Bx = txi * TW, By = tyi * TH;
         const int e0 = it.y, e1 = it.y + it.z;
-
+        // Items whose entries may use the table's first or last slice (entries
+        // are ordered by C-slice key, so only the first and last entries need
+        // looking at) load every pair inside its slice (EDGE); the others read
+        // straddling pairs whole.  Deciding per item rather than per entry keeps
+        // the entry loop free of a branch whose two paths would make the
+        // compiler copy the accumulators on every entry.
+        const bool edgeItem = cload(entries, e0).y < jd.cEdgeLo || cload(entries, e1 - 1).y > jd.cEdgeHi;
+        auto run = [&](auto edgeTag) {
+        constexpr bool EDGE = decltype(edgeTag)::value;
         float ar0[R], ai0[R], ar1[R], ai1[R];
 #pragma unroll
         for (int j = 0; j < R; ++j) { ar0[j] = ai0[j] = ar1[j] = ai1[j] = 0.f; }
@@ -1082,7 +1089,7 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled2(ItemSource<BATCH> src)
             const int dy = By - v0;
             const int cyl = ly + dy;       // C row of the lane's row in group 0, less r0
             const int sTB = s * (int)TB;
-            if (c0 > 0 && c0 + s * s < nTaps) {
+            if constexpr (!EDGE) {
                 // the pair (cx, cx + 1) even where it straddles the footprint's edge:
                 // the outside tap is the neighbouring one in the table, and its cell
                 // takes it with a zero visibility (an exact zero, tables are finite)
@@ -1197,6 +1204,9 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled2(ItemSource<BATCH> src)
                 __builtin_amdgcn_wave_barrier();
             }
         }
+        };
+        if (edgeItem) run(std::true_type{});
+        else run(std::false_type{});
     }
 }
 
@@ -2177,6 +2187,10 @@ JobDesc make_job_desc(const tcv_plan *p, bool forGrid, const void *data, const v
     d.planeTaps = half ? half->planeTaps : 0;
     d.dataScale = half ? half->dataScale : 1.f;
     d.nVis = p->nVis;
+    // one C-slice key spans cSize / k2 taps; two more for the rounding of the key
+    const long long keyTaps = (p->cSize + p->sh.k2 - 1) / p->sh.k2 + 2;
+    d.cEdgeLo = (int)std::min<long long>(keyTaps, 0x7fffffffLL);
+    d.cEdgeHi = (int)std::max<long long>(p->cSize - (long long)p->sh.maxS * p->sh.maxS - keyTaps, -1);
     return d;
 }
 
