@@ -785,27 +785,33 @@ __device__ __forceinline__ void load_tile(float2 *tile, const float2 *__restrict
 {
     constexpr int kPairs = TH * TW / 2;
     constexpr int kPer = (kPairs + 255) / 256;
-    float4 v[kPer];
+    constexpr int kBlk = kPer < 8 ? kPer : 8;   // loads in flight per thread (register bound)
 #pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-        const int q = 2 * ((int)threadIdx.x + 256 * i);   // first cell of the pair
-        const int row = By + q / TW, col = Bx + q % TW;
-        v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (q < TH * TW && row < gSize) {
-            const float2 *g = grid + (size_t)row * gSize + col;
-            if (col + 1 < gSize) {
-                __builtin_memcpy(&v[i], g, 16);   // 8-byte aligned when gSize is odd
-            } else if (col < gSize) {
-                const float2 a = *g;
-                v[i].x = a.x;
-                v[i].y = a.y;
+    for (int i0 = 0; i0 < kPer; i0 += kBlk) {
+        float4 v[kBlk];
+#pragma unroll
+        for (int b = 0; b < kBlk; ++b) {
+            const int q = 2 * ((int)threadIdx.x + 256 * (i0 + b));   // first cell of the pair
+            const int row = By + q / TW, col = Bx + q % TW;
+            v[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (i0 + b < kPer && q < TH * TW && row < gSize) {
+                const float2 *g = grid + (size_t)row * gSize + col;
+                if (col + 1 < gSize) {
+                    __builtin_memcpy(&v[b], g, 16);   // 8-byte aligned when gSize is odd
+                } else if (col < gSize) {
+                    const float2 a = *g;
+                    v[b].x = a.x;
+                    v[b].y = a.y;
+                }
             }
         }
-    }
 #pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-        const int q = 2 * ((int)threadIdx.x + 256 * i);
-        if (q < TH * TW) *reinterpret_cast<float4 *>(tile + q) = v[i];
+        for (int b = 0; b < kBlk; ++b) {
+            const int q = 2 * ((int)threadIdx.x + 256 * (i0 + b));
+            if (i0 + b < kPer && q < TH * TW) *reinterpret_cast<float4 *>(tile + q) = v[b];
+        }
+        // keep the next block's loads after these stores
+        if constexpr (kPer > kBlk) __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -1161,22 +1167,28 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled2(ItemSource<BATCH> src)
         if (it.w) {
             // the tile is this item's alone: plain read-modify-write, every read
             // issued before the first write so that their latencies overlap
-            float2 o0[R], o1[R];
+            // (RB rows at a time: the old values' registers)
 #pragma unroll
-            for (int j = 0; j < R; ++j) {
-                const float2 *gp = g0 + (size_t)j * RPI * gSize;
-                const bool rok = j * RPI < rowsLeft;
-                o0[j] = (rok && c0ok) ? gp[0] : make_float2(0.f, 0.f);
-                o1[j] = (rok && c1ok) ? gp[1] : make_float2(0.f, 0.f);
-            }
+            for (int j0 = 0; j0 < R; j0 += RB) {
+                float2 o0[RB], o1[RB];
 #pragma unroll
-            for (int j = 0; j < R; ++j) {
-                float2 *gp = g0 + (size_t)j * RPI * gSize;
-                const bool rok = j * RPI < rowsLeft;
-                if (rok && c0ok && (ar0[j] != 0.f || ai0[j] != 0.f))
-                    gp[0] = make_float2(o0[j].x + ar0[j], o0[j].y + ai0[j]);
-                if (rok && c1ok && (ar1[j] != 0.f || ai1[j] != 0.f))
-                    gp[1] = make_float2(o1[j].x + ar1[j], o1[j].y + ai1[j]);
+                for (int b = 0; b < RB; ++b) {
+                    const int j = j0 + b;
+                    const float2 *gp = g0 + (size_t)j * RPI * gSize;
+                    const bool rok = j * RPI < rowsLeft;
+                    o0[b] = (rok && c0ok) ? gp[0] : make_float2(0.f, 0.f);
+                    o1[b] = (rok && c1ok) ? gp[1] : make_float2(0.f, 0.f);
+                }
+#pragma unroll
+                for (int b = 0; b < RB; ++b) {
+                    const int j = j0 + b;
+                    float2 *gp = g0 + (size_t)j * RPI * gSize;
+                    const bool rok = j * RPI < rowsLeft;
+                    if (rok && c0ok && (ar0[j] != 0.f || ai0[j] != 0.f))
+                        gp[0] = make_float2(o0[b].x + ar0[j], o0[b].y + ai0[j]);
+                    if (rok && c1ok && (ar1[j] != 0.f || ai1[j] != 0.f))
+                        gp[1] = make_float2(o1[b].x + ar1[j], o1[b].y + ai1[j]);
+                }
             }
         } else {
             // Atomic flush with every instruction on contiguous floats: each row
@@ -2077,9 +2089,11 @@ constexpr int kRB2 = TCV_RB2;
 
 template <int TW, int TH>
 struct TileCfg {
-    static constexpr int RPI = 64 / TW;
-    static constexpr int RG = TH / (kNW * RPI);   // gridder rows groups per wave
-    static constexpr int RD = TH / RPI;           // degridder row groups per wave
+    // tiles wider than a wave have only the pair kernels (kernel 3)
+    static constexpr bool kPairOnly = TW > 64;
+    static constexpr int RPI = kPairOnly ? 1 : 64 / TW;
+    static constexpr int RG = kPairOnly ? 0 : TH / (kNW * RPI);   // gridder rows groups per wave
+    static constexpr int RD = kPairOnly ? 0 : TH / RPI;           // degridder row groups per wave
     static constexpr int RG2 = TH / (kNW * (128 / TW));   // two-column gridder (0: not built)
     static constexpr int RG4 = TH * TW / (kNW * 256);      // four-column FP16 gridder (0: not built)
 };
@@ -2111,13 +2125,15 @@ template <int TW, int TH, bool HALF, bool BATCH>
 tcv_status grid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_kind)
 {
     using T = TileCfg<TW, TH>;
-    auto kern = kern_kind == 2 ? k_grid_lds<TW, TH, HALF, BATCH>
-                               : k_grid_tiled<TW, T::RG, kNW, (T::RG % kRB == 0 ? kRB : T::RG), HALF, BATCH>;
+    void (*kern)(ItemSource<BATCH>) = nullptr;
+    if constexpr (!T::kPairOnly)
+        kern = kern_kind == 2 ? k_grid_lds<TW, TH, HALF, BATCH>
+                              : k_grid_tiled<TW, T::RG, kNW, (T::RG % kRB == 0 ? kRB : T::RG), HALF, BATCH>;
     if constexpr (T::RG2 >= 1) {
-        if (kern_kind >= 3)
+        if (kern_kind >= 3 || T::kPairOnly)
             kern = k_grid_tiled2<TW, T::RG2, kNW, (T::RG2 % kRB2 == 0 ? kRB2 : T::RG2), HALF, BATCH>;
     }
-    if constexpr (HALF && T::RG4 >= 1) {
+    if constexpr (HALF && T::RG4 >= 1 && !T::kPairOnly) {
         // FP16 storage: four taps per 16-byte load
         if (kern_kind == 4)
             kern = k_grid_tiled4h<TW, T::RG4, kNW, (T::RG4 % kRB == 0 ? kRB : T::RG4), BATCH>;
@@ -2133,10 +2149,13 @@ template <int TW, int TH, bool HALF, bool BATCH>
 tcv_status degrid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_kind)
 {
     using T = TileCfg<TW, TH>;
-    auto kern = kern_kind == 2 ? k_degrid_lds<TW, TH, HALF, BATCH>
-                               : k_degrid_tiled<TW, T::RD, kNW, (T::RD % kRB == 0 ? kRB : T::RD), HALF, BATCH>;
-    if (kern_kind >= 3) kern = k_degrid_lds2<TW, TH, HALF, BATCH>;
-    if constexpr (HALF) {
+    void (*kern)(ItemSource<BATCH>) = k_degrid_lds2<TW, TH, HALF, BATCH>;
+    if constexpr (!T::kPairOnly) {
+        if (kern_kind == 2) kern = k_degrid_lds<TW, TH, HALF, BATCH>;
+        else if (kern_kind < 3)
+            kern = k_degrid_tiled<TW, T::RD, kNW, (T::RD % kRB == 0 ? kRB : T::RD), HALF, BATCH>;
+    }
+    if constexpr (HALF && !T::kPairOnly) {
         if (kern_kind == 4) kern = k_degrid_lds4h<TW, TH, BATCH>;   // four FP16 taps per load
     }
     const int blocks = persistent_blocks(ctx, (const void *)kern, kNW * 64);
@@ -2149,7 +2168,7 @@ tcv_status degrid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_k
 }  // namespace
 
 #define TCV_TILE_CONFIGS(X) \
-    X(64, 64) X(64, 32) X(32, 64) X(32, 32) X(16, 64) X(16, 32) X(8, 64) X(8, 32)
+    X(128, 64) X(64, 64) X(64, 32) X(32, 64) X(32, 32) X(16, 64) X(16, 32) X(8, 64) X(8, 32)
 
 bool plan_two_level(const PlanShape &sh)
 {

@@ -192,8 +192,10 @@ tcv_status tcv_range_errors(tcv_ctx *ctx, long long *count);
 /* ---- a7/a8 tiled path: visibilities binned by grid tile on the device ------- */
 
 typedef struct {
-    int tile_w;      /* 8, 16, 32 or 64 columns (0 = auto) */
-    int tile_h;      /* rows per tile (multiple of 64/tile_w; 0 = auto) */
+    int tile_w;      /* 8, 16, 32, 64 or 128 columns (0 = auto: 128 x 64 tiles for
+                        kernels wider than 128 when the pair kernels apply); 128-wide
+                        tiles take only the pair kernels (kernel and degrid_kernel 3) */
+    int tile_h;      /* rows per tile: 32 or 64 (64 for 128-wide tiles; 0 = auto) */
     int chunk;       /* max entries per work item (0 = auto) */
     int slice_keys;  /* order entries in a tile by C-slice key, 0 = no ordering */
     int phases;      /* >1: split each tile's entries into C-slice phases (0 = auto) */

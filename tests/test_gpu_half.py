@@ -92,6 +92,30 @@ def test_half_config1_errors_reported(ctx):
     assert e32 <= 1e-5 and e16 <= HALF_TOL
 
 
+@pytest.mark.parametrize("Sz", [5, 129])
+def test_half_wide_tiles(ctx, Sz):
+    """FP16 storage on 128 x 64 tiles (two-tap pair kernels): within the FP16
+    tolerance of the FP64 oracle; the four-tap code is refused on such tiles."""
+    gSize, os_, planes = 300, 4, 3
+    iu, iv, co, data, C = small_problem(720 + Sz, 1500, gSize, Sz, os_=os_, nPlanes=planes,
+                                        cluster=60)
+    vis = tcv.Visibilities(ctx, iu, iv, co, [Sz], gSize, C.size)
+    d_C, d_data = ctx.to_device(C), ctx.to_device(data)
+    Ch = tcv.HalfTable(ctx, d_C, planes, Sz * Sz * os_ * os_)
+    dh = tcv.HalfVis(ctx, d_data)
+    plan = tcv.Plan(ctx, vis, tile_w=128, tile_h=64, kernel=3, degrid_kernel=3, phases=2,
+                    slice_keys=8, chunk=64)
+    g = ctx.zeros((gSize, gSize), np.complex64)
+    plan.grid_half(dh, Ch, g)
+    g64 = O.grid_f64(data, iu, iv, co, C, gSize, Sz)
+    assert rel_err(g.download(), g64) <= HALF_TOL
+    out = ctx.empty(vis.nVis, np.complex64)
+    plan.degrid_half(ctx.to_device(g64.astype(np.complex64)), Ch, out)
+    assert rel_err(out.download(), O.degrid_f64(g64, iu, iv, co, C, Sz)) <= HALF_TOL
+    with pytest.raises(tcv.TcvError):
+        tcv.Plan(ctx, vis, tile_w=128, tile_h=64, kernel=4, degrid_kernel=4)
+
+
 @pytest.mark.parametrize("Sz", [3, 5, 33])
 @pytest.mark.parametrize("tw,th", [(64, 64), (32, 32), (16, 32), (8, 64)])
 def test_half_four_tap_kernels_shapes(ctx, Sz, tw, th):
