@@ -440,10 +440,10 @@ tcv_status tcv_range_errors(tcv_ctx *ctx, long long *count)
 // per (tile, phase) work item on average: every item zeroes and flushes its
 // tile.  From 8 phases up a multiple of 8, so the phases split evenly over
 // the XCDs.
-static int default_phases(long long cSize, int sliceKeys, const tcv_geometry *g, int maxS, int tw,
-                          int th, double bytesPerPhase)
+static int default_phases(long long cSize, int tapBytes, int sliceKeys, const tcv_geometry *g, int maxS,
+                          int tw, int th, double bytesPerPhase)
 {
-    double target = std::max(1.0, cSize * 8.0 / bytesPerPhase);
+    double target = std::max(1.0, (double)cSize * tapBytes / bytesPerPhase);
     if (g && tw > 0 && th > 0) {
         const double perVis = (1.0 + (maxS - 1.0) / tw) * (1.0 + (maxS - 1.0) / th);
         const double tiles = (double)((g->gSize + tw - 1) / tw) * ((g->gSize + th - 1) / th);
@@ -473,10 +473,12 @@ static void auto_tuning(int maxS, const tcv_geometry *g, tcv_tuning *t)
     // tiles: each footprint row then meets two tiles instead of three and each
     // visibility makes a third fewer entries.  Measured 9 % faster at width 129,
     // 2-5 % slower at widths 113 and 121, where 64-wide tiles are kept.
+    // (FP16 storage: 64-wide tiles measured 18 % faster than 128-wide at width 129)
+    if (t->tap_bytes <= 0) t->tap_bytes = 8;
     const bool pairs = minS >= 2 && (t->kernel <= 0 || t->kernel == 3) &&
                        (t->degrid_kernel <= 0 || t->degrid_kernel == 3);
     if (t->tile_w <= 0)
-        t->tile_w = (maxS > 128 && pairs && t->tile_h <= 64) ? 128
+        t->tile_w = (maxS > 128 && pairs && t->tile_h <= 64 && t->tap_bytes == 8) ? 128
                     : maxS >= 48 ? 64 : maxS >= 24 ? 32 : maxS >= 12 ? 16 : 8;
     if (t->tile_h <= 0) t->tile_h = 64;
     if (t->chunk <= 0) t->chunk = 2048;
@@ -484,12 +486,12 @@ static void auto_tuning(int maxS, const tcv_geometry *g, tcv_tuning *t)
     if (t->slice_keys <= 0) t->slice_keys = (int)std::min<long long>(2048, slices);
     const bool wide = t->tile_w > 64;
     if (t->phases <= 0)
-        t->phases = default_phases(g->cSize, t->slice_keys, g, maxS, t->tile_w, t->tile_h,
+        t->phases = default_phases(g->cSize, t->tap_bytes, t->slice_keys, g, maxS, t->tile_w, t->tile_h,
                                    wide ? kWidePhaseBytes : kDegridPhaseBytes);
     if (t->xcd_split < 0) t->xcd_split = (t->phases >= 8 && t->phases % 8 == 0) ? 1 : 0;
     if (t->grid_phases <= 0)
-        t->grid_phases = default_phases(g->cSize, t->slice_keys, g, maxS, t->tile_w, t->tile_h,
-                                        wide ? kWidePhaseBytes : kGridPhaseBytes);
+        t->grid_phases = default_phases(g->cSize, t->tap_bytes, t->slice_keys, g, maxS, t->tile_w,
+                                        t->tile_h, wide ? kWidePhaseBytes : kGridPhaseBytes);
     if (t->kernel <= 0) t->kernel = kDefaultGridKernel;
     if (t->degrid_kernel <= 0) t->degrid_kernel = kDefaultDegridKernel;
 }
@@ -513,7 +515,9 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
     if (g->gSize > 65535) return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tiled path needs gSize <= 65535");
     if (g->cSize * 8 >= 0x7fffffffLL)
         return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tiled path needs a C table below 2 GiB");
-    tcv_tuning t = tuning ? *tuning : tcv_tuning{0, 0, 0, 0, 0, -1, 0, 0, 0};
+    tcv_tuning t = tuning ? *tuning : tcv_tuning{0, 0, 0, 0, 0, -1, 0, 0, 0, 0};
+    if (t.tap_bytes != 0 && t.tap_bytes != 4 && t.tap_bytes != 8)
+        return tcv_fail(ctx, TCV_ERR_INVALID, "tuning.tap_bytes must be 0, 4 or 8");
     if (!tuning && ctx) {
         auto it = ctx->tuned.find(tuning_class(g));
         if (it != ctx->tuned.end()) t = it->second;
@@ -536,6 +540,7 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
         return tcv_fail(ctx, TCV_ERR_INVALID, "tuning.kernel and degrid_kernel must be 0..4");
     sh.kern = t.kernel;
     sh.dkern = t.degrid_kernel;
+    sh.tapBytes = t.tap_bytes;
     // the pair kernels need kernel widths of at least 2 (a pair inside each C row)
     int minS = maxS;
     for (int k = 0; k < g->nPlanes; k++) minS = std::min(minS, g->sSize[k]);
@@ -706,7 +711,7 @@ tcv_status tcv_plan_info(tcv_ctx *ctx, tcv_plan *p, tcv_tuning *t, long long *en
 {
     if (!ctx || !p) return TCV_ERR_INVALID;
     if (t) *t = tcv_tuning{p->sh.tw, p->sh.th, p->sh.chunk, p->sh.k2, p->sh.phases, p->sh.xcd, p->sh.kern,
-                           p->sh.dkern, p->sh.phasesG};
+                           p->sh.dkern, p->sh.phasesG, p->sh.tapBytes};
     int meta[4] = {0, 0, 0, 0};   // items, queue head, entries, out of range
     TCV_HIP(ctx, hipMemcpyAsync(meta, p->meta, sizeof(meta), hipMemcpyDeviceToHost, ctx->stream));
     TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1034,7 +1039,7 @@ tcv_status tcv_autotune(tcv_ctx *ctx, const tcv_geometry *g, const float *d_data
     } else {
         // staged search from the defaults: tile shape x kernel family, then the
         // phase counts of the degridder and gridder items, then the item size
-        tcv_tuning d{0, 0, 0, 0, 0, -1, 0, 0, 0};
+        tcv_tuning d{0, 0, 0, 0, 0, -1, 0, 0, 0, 0};
         auto_tuning(maxS, g, &d);
         int minS = maxS;
         for (int k = 0; k < g->nPlanes; k++) minS = std::min(minS, g->sSize[k]);

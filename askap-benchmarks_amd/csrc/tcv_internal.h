@@ -124,6 +124,7 @@ struct PlanShape {
     int kern, dkern;       // gridder / degridder kernel (codes as in tcv_tuning)
     int phasesG, xcdG;     // the gridder's work-item phases and XCD order
     int minS;              // narrowest kernel width of the job
+    int tapBytes;          // stored tap size the tuning was chosen for (8 or 4)
     int nTX, nTY, nTiles;
     int maxS;
     long long maxEntries;  // upper bound on visibility x tile pairs

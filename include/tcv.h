@@ -211,6 +211,10 @@ typedef struct {
                         storage; 0 = auto */
     int grid_phases; /* C-slice phases of the gridder's work items (the degridder uses
                         phases; both cut the same sorted entries); 0 = auto */
+    int tap_bytes;   /* bytes per stored C tap the automatic choices are made for: 8 FP32
+                        storage (0 = 8), 4 FP16 storage (tcv_grid_tiled_half and
+                        tcv_degrid_tiled_half); it sets the default tile shape and phase
+                        counts, which follow the table's size in bytes */
 } tcv_tuning;
 
 typedef struct tcv_plan tcv_plan;

@@ -92,6 +92,34 @@ def test_half_config1_errors_reported(ctx):
     assert e32 <= 1e-5 and e16 <= HALF_TOL
 
 
+def test_half_plan_defaults_for_four_byte_taps(ctx):
+    """A plan made for FP16 storage (tap_bytes=4) keeps 64-wide tiles at width
+    129 and cuts the table into half as many phases as an FP32 plan (the same
+    bytes per phase); its FP16 results stay within the FP16 tolerance, and a
+    tap size other than 0, 4 or 8 is refused."""
+    Sz, gSize, os_, planes = 129, 400, 8, 5
+    iu, iv, co, data, C = small_problem(730, 3000, gSize, Sz, os_=os_, nPlanes=planes)
+    vis = tcv.Visibilities(ctx, iu, iv, co, [Sz], gSize, C.size)
+    i32 = tcv.Plan(ctx, vis).info()
+    plan = tcv.Plan(ctx, vis, tap_bytes=4)
+    i16 = plan.info()
+    assert (i32["tile_w"], i32["tap_bytes"]) == (128, 8)
+    assert (i16["tile_w"], i16["tile_h"], i16["tap_bytes"]) == (64, 64, 4)
+    assert i16["phases"] <= i32["phases"]
+    d_C, d_data = ctx.to_device(C), ctx.to_device(data)
+    Ch = tcv.HalfTable(ctx, d_C, planes, Sz * Sz * os_ * os_)
+    dh = tcv.HalfVis(ctx, d_data)
+    g = ctx.zeros((gSize, gSize), np.complex64)
+    plan.grid_half(dh, Ch, g)
+    g64 = O.grid_f64(data, iu, iv, co, C, gSize, Sz)
+    assert rel_err(g.download(), g64) <= HALF_TOL
+    out = ctx.empty(vis.nVis, np.complex64)
+    plan.degrid_half(ctx.to_device(g64.astype(np.complex64)), Ch, out)
+    assert rel_err(out.download(), O.degrid_f64(g64, iu, iv, co, C, Sz)) <= HALF_TOL
+    with pytest.raises(tcv.TcvError):
+        tcv.Plan(ctx, vis, tap_bytes=2)
+
+
 @pytest.mark.parametrize("Sz", [5, 129])
 def test_half_wide_tiles(ctx, Sz):
     """FP16 storage on 128 x 64 tiles (two-tap pair kernels): within the FP16

@@ -109,3 +109,28 @@ def test_spectral_workload_fits_the_grid():
     assert (iv >= 0).all() and (iv + s <= prob.gSize).all()
     assert (co >= 0).all() and (co.astype(np.int64) + s * s <= prob.C.size).all()
     assert abs(np.abs(prob.data) ** 2).mean() == pytest.approx(1.0, rel=0.02)
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every ctypes mirror of a C-ABI struct has the header's field names,
+    offsets and size (compiled with gcc against include/tcv.h)."""
+    from tcv import _lib as L
+    pairs = [(L.Geometry, "tcv_geometry"), (L.Job, "tcv_job"), (L.ServerConfig, "tcv_server_config"),
+             (L.Request, "tcv_request"), (L.Result, "tcv_result"), (L.Tuning, "tcv_tuning")]
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "tcv.h"', "int main(void) {"]
+    want = []
+    for cls, cname in pairs:
+        lines.append(f'  printf("%zu\\n", sizeof({cname}));')
+        want.append(ct.sizeof(cls))
+        for name, _ in cls._fields_:
+            lines.append(f'  printf("%zu\\n", offsetof({cname}, {name}));')
+            want.append(getattr(cls, name).offset)
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-std=c11", "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    assert got == want
