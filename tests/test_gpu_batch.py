@@ -97,11 +97,13 @@ def test_device_initC_matches_host_taps(ctx):
     assert np.abs(d.astype(np.complex128)).sum() == pytest.approx(33 * 64, rel=1e-6)
 
 
-@pytest.mark.parametrize("kernels", [(2, 2), (3, 3), (1, 3)])
+@pytest.mark.parametrize("kernels", [(2, 2, 64), (3, 3, 64), (1, 3, 64), (3, 3, 128)])
 def test_batch_other_kernels_vs_oracle(ctx, kernels):
-    # gridder and degridder items cut at different phase counts
+    # gridder and degridder items cut at different phase counts; 128 x 64 tiles
+    # (pair kernels) through the batched launches too
     bj = BatchJobs(ctx, small_specs(4, seed=21), tuning=dict(kernel=kernels[0],
                                                               degrid_kernel=kernels[1],
+                                                              tile_w=kernels[2],
                                                               phases=8, grid_phases=2))
     bj.zero_grids()
     bj.run_batched(graph=True)
