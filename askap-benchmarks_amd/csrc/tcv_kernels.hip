@@ -1034,7 +1034,7 @@ __global__ __launch_bounds__(NW * 64) void k_degrid_tiled(ItemSource<BATCH> src)
 // so every load stays inside the visibility's C slice.  Same per-cell
 // arithmetic and entry order as the kernels above.
 template <int TW, int R, int NW, int RB, bool HALF, bool BATCH>
-__global__ __launch_bounds__(NW * 64) void k_grid_tiled2(ItemSource<BATCH> src)
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) void k_grid_tiled2(ItemSource<BATCH> src)
 {
     constexpr int LPR = TW / 2;      // lanes per tile row
     constexpr int RPI = 64 / LPR;    // tile rows per instruction

@@ -100,7 +100,18 @@ class GridJob:
         return self.iu.download(), self.iv.download(), wp, self.co.download()
 
 
-__all__ = ["init_offsets", "GridJob", "TcvError"]
+__all__ = ["init_offsets", "GridJob", "TcvError", "batch_tile_w"]
+
+
+def batch_tile_w(widths, tuning: dict | None = None) -> int:
+    """The one tile width shared by a batch (or a server's requests): 128 when
+    the widest kernel is wider than 128 (the rule a single job's plan follows),
+    every kernel is at least 2 wide and the pair kernels (code 3, or automatic)
+    are used; else 64."""
+    t = tuning or {}
+    pairs = t.get("kernel", 0) in (0, 3) and t.get("degrid_kernel", 0) in (0, 3)
+    widths = [int(w) for w in widths]
+    return 128 if pairs and max(widths) > 128 and min(widths) >= 2 else 64
 
 
 class BatchJobs:
@@ -121,7 +132,7 @@ class BatchJobs:
         self.specs = specs
         self.jobs = []
         tuning = dict(tuning or {})
-        tuning.setdefault("tile_w", 64)
+        tuning.setdefault("tile_w", batch_tile_w([sp["sSize"] for sp in specs], tuning))
         for sp in specs:
             sup, wcs, C = attic_initC_device(ctx, float(sp["freq"][0]), sp["cellSize"], sp["wSize"],
                                              int(sp["baseline"]))

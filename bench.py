@@ -276,7 +276,7 @@ def serving_setup(ctx):
     budget = 200 * 10**9
     arena = int(min(budget - resident, free - 16 * 2**30))
     mean_pts = float(np.mean([sp["nVis"] * sp["sSize"] ** 2 for sp in specs]))
-    return bj, arena, resident, budget, mean_pts
+    return bj, arena, resident, budget, mean_pts, specs
 
 
 def bench_serving(ctx, args, capacity):
@@ -284,13 +284,16 @@ def bench_serving(ctx, args, capacity):
     specified rate, and a declared rate at 60% of the measured batched
     capacity.  Requests follow the uneven 64-job size distribution."""
     import tcv
-    bj, arena, resident, budget, mean_pts = serving_setup(ctx)
+    from tcv.gridder import batch_tile_w
+    bj, arena, resident, budget, mean_pts, specs = serving_setup(ctx)
+    tile_w = batch_tile_w([sp["sSize"] for sp in specs])
     srv = tcv.Server(ctx, arena, max_inflight=args.serve_inflight, auto_release=True,
-                     max_batch_points=int(args.serve_batch_points))
+                     max_batch_points=int(args.serve_batch_points), tile_w=tile_w, tile_h=64)
     out = {"workload": "Poisson requests over the 64 uneven jobs, 50% grid into new grids, 50% degrid "
                        "resident grids, HBM arena admission, overlapping batches",
            "arena_bytes": arena, "resident_bytes": resident, "budget_bytes": budget,
            "max_inflight": args.serve_inflight, "max_batch_points": args.serve_batch_points,
+           "tile": [tile_w, 64],
            "capacity_points_per_s": capacity, "mean_points_per_request": mean_pts}
     out["specified"] = serving_stream(srv, bj, mean_pts, capacity, 200.0, args.serve_requests, 1)
     log(f"serving @200/s: p50 {out['specified']['p50_ms']:.1f} ms p99 {out['specified']['p99_ms']:.1f} ms, "
