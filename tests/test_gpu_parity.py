@@ -432,11 +432,13 @@ def test_pair_kernels_every_width_exact(ctx, Sz):
         assert np.array_equal(gpu_degrid(ctx, vis, G, C, "tiled", tuning), oref), (tw, th)
 
 
-@pytest.mark.parametrize("Sz", [3, 9, 33, 129])
-def test_pair_kernels_first_and_last_slice_exact(ctx, Sz):
+@pytest.mark.parametrize("Sz,keys", [(3, 4), (9, 4), (33, 4), (129, 4), (9, 4096), (33, 1)])
+def test_pair_kernels_first_and_last_slice_exact(ctx, Sz, keys):
     """Footprints whose C slice is the table's first or last, at every offset
     around a tile edge: the pair kernels then load the pair that starts or ends
-    on the edge tap (no tap outside the table is read); bit-exact."""
+    on the edge tap (no tap outside the table is read); bit-exact.  The
+    gridder chooses that path per work item from its first and last entries'
+    C-slice keys: also with more keys than slices (4096) and with one key."""
     gSize = 256
     iu, iv, co, data, C, G = exact_problem(500 + Sz, 400, gSize, Sz, cluster=None)
     half = Sz // 2
@@ -456,7 +458,8 @@ def test_pair_kernels_first_and_last_slice_exact(ctx, Sz):
     gref = O.grid(data, iu, iv, co, C, gSize, Sz)
     oref = O.degrid(G, iu, iv, co, C, Sz)
     for tw, th in ((128, 64), (64, 64), (32, 64), (16, 32)):
-        tuning = dict(tile_w=tw, tile_h=th, kernel=3, degrid_kernel=3, phases=2, slice_keys=4, chunk=48)
+        tuning = dict(tile_w=tw, tile_h=th, kernel=3, degrid_kernel=3, phases=min(2, keys),
+                      slice_keys=keys, chunk=48)
         assert np.array_equal(gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning), gref), (tw, th)
         assert np.array_equal(gpu_degrid(ctx, vis, G, C, "tiled", tuning), oref), (tw, th)
 
