@@ -856,8 +856,8 @@ tcv_status tcv_batch_create(tcv_ctx *ctx, int nJobs, const tcv_job *jobs, tcv_ba
     };
     if (hipMalloc((void **)&b->d_jobs, sizeof(JobDesc) * nJobs) != hipSuccess ||
         hipMalloc((void **)&b->d_jobsD, sizeof(JobDesc) * nJobs) != hipSuccess ||
-        hipMalloc((void **)&b->d_prefix, sizeof(int) * (nJobs + 1)) != hipSuccess ||
-        hipMalloc((void **)&b->d_queue, 256) != hipSuccess)
+        hipMalloc((void **)&b->d_prefix, sizeof(int) * 9 * (nJobs + 1)) != hipSuccess ||
+        hipMalloc((void **)&b->d_queue, kBatchQueueBytes) != hipSuccess)
         return fail("tcv_batch_create: allocation failed");
     if (hipMemcpy(b->d_jobs, b->host.data(), sizeof(JobDesc) * nJobs, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->d_jobsD, b->hostD.data(), sizeof(JobDesc) * nJobs, hipMemcpyHostToDevice) != hipSuccess)

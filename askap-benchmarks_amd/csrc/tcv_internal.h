@@ -111,6 +111,7 @@ struct JobDesc {
 };
 
 constexpr int kMaxPersistentBlocks = 8192;
+constexpr size_t kBatchQueueBytes = 1024;   // ints 0 and 16 + 16 q (q < 8) are used
 
 // Tiled kernels used when a tuning leaves the choice open.  Gridder: 1 tile in
 // registers, 2 tile in LDS, 3 registers with two columns per lane.  Degridder:
@@ -178,8 +179,8 @@ struct tcv_batch {
     std::vector<JobDesc> host, hostD;   // gridder / degridder descriptors
     std::vector<tcv_plan *> plans;
     JobDesc *d_jobs = nullptr, *d_jobsD = nullptr;
-    int *d_prefix = nullptr;      // nJobs + 1
-    int *d_queue = nullptr;
+    int *d_prefix = nullptr;      // 9 x (nJobs + 1): item prefix, then one per XCD
+    int *d_queue = nullptr;       // kBatchQueueBytes: the queue and the per-XCD counters
     hipGraphExec_t graph = nullptr;
     int graphOps = 0;
 };

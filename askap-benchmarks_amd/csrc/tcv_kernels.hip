@@ -704,7 +704,10 @@ struct ItemSource {
     int nJobs;
     int *queue;
 
-    int xcd;   // single job: items are XCD-major with per-XCD queues
+    // Single job: items are XCD-major with per-XCD queues.  Batch: each XCD
+    // walks its share of every job in job order (per-XCD prefix sums after the
+    // job prefix, counters at queue[16 + 16 q]), then takes from the others.
+    int xcd;
     int *bcast;   // one slot per workgroup (kernels that keep LDS for their tile)
 
     __device__ __forceinline__ int total() const
@@ -715,9 +718,27 @@ struct ItemSource {
     // next work item (one lane calls this); >= total() when none is left
     __device__ __forceinline__ int acquire() const
     {
-        if (BATCH || !xcd) return atomicAdd(queue, 1);
+        if (!xcd) return atomicAdd(queue, 1);
         // own XCD first (hardware XCC_ID register), then steal from the others
         const int x0 = (int)(__builtin_amdgcn_s_getreg((3 << 11) | 20) & 7);
+        if constexpr (BATCH) {
+            const int n1 = nJobs + 1;
+            for (int k = 0; k < 8; ++k) {
+                const int q = (x0 + k) & 7;
+                const int *px = prefix + n1 * (1 + q);
+                const int tot = px[nJobs];
+                if (tot == 0) continue;
+                const int i = atomicAdd(queue + 16 + 16 * q, 1);
+                if (i >= tot) continue;
+                int lo = 0, hi = nJobs;   // px[lo] <= i < px[hi]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (px[mid] <= i) lo = mid; else hi = mid;
+                }
+                return prefix[lo] + jobs[lo].meta[16 + q] + (i - px[lo]);
+            }
+            return 0x7fffffff;
+        }
         for (int k = 0; k < 8; ++k) {
             const int q = (x0 + k) & 7;
             const int b = one.meta[16 + q], e = one.meta[17 + q];
@@ -2107,18 +2128,22 @@ int persistent_blocks(tcv_ctx *ctx, const void *kernel, int bs)
     return std::min(std::max(1, ctx->cus) * per, kMaxPersistentBlocks);
 }
 
+// Item prefix over the jobs (thread 0) and, after it, one prefix per XCD over
+// the jobs' per-XCD item ranges (threads 1..8); resets the queue counters.
 __global__ void k_batch_prefix(const JobDesc *__restrict__ jobs, int nJobs, int *__restrict__ prefix,
                                int *__restrict__ queue)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        int run = 0;
-        for (int j = 0; j < nJobs; ++j) {
-            prefix[j] = run;
-            run += jobs[j].meta[0];
-        }
-        prefix[nJobs] = run;
-        *queue = 0;
+    const int t = threadIdx.x;
+    if (blockIdx.x != 0 || t > 8) return;
+    int *out = prefix + (nJobs + 1) * t;
+    int run = 0;
+    for (int j = 0; j < nJobs; ++j) {
+        out[j] = run;
+        const int *m = jobs[j].meta;
+        run += t == 0 ? m[0] : m[16 + t] - m[15 + t];
     }
+    out[nJobs] = run;
+    queue[16 * t] = 0;   // t = 0: the single queue; t = 1 + q: XCD q's counter
 }
 
 template <int TW, int TH, bool HALF, bool BATCH>
@@ -2263,7 +2288,7 @@ tcv_status launch_batch(tcv_ctx *ctx, tcv_batch *b, bool degrid)
                        b->d_queue);
     TCV_HIP(ctx, hipGetLastError());
     ItemSource<true> src{JobDesc{}, degrid ? b->d_jobsD : b->d_jobs, b->d_prefix, b->nJobs,
-                         b->d_queue, 0, ctx->d_bcast};
+                         b->d_queue, 1, ctx->d_bcast};
     // batches run FP32 storage
 #define X(a, c)                                                                        \
     if (b->tw == a && b->th == c)                                                      \

@@ -293,8 +293,8 @@ tcv_status tcv_server_create(tcv_ctx *ctx, const tcv_server_config *cfg, tcv_ser
             b->kern = s->tuning.kernel;
             b->dkern = s->tuning.degrid_kernel;
             if (hipMalloc((void **)&b->d_jobs, sizeof(JobDesc) * maxJobs) != hipSuccess ||
-                hipMalloc((void **)&b->d_prefix, sizeof(int) * (maxJobs + 1)) != hipSuccess ||
-                hipMalloc((void **)&b->d_queue, 256) != hipSuccess)
+                hipMalloc((void **)&b->d_prefix, sizeof(int) * 9 * (maxJobs + 1)) != hipSuccess ||
+                hipMalloc((void **)&b->d_queue, kBatchQueueBytes) != hipSuccess)
                 st = TCV_ERR_NOMEM;
         }
         if (st == TCV_OK && hipHostMalloc((void **)&sl.pinned, sizeof(JobDesc) * maxJobs * 2) != hipSuccess)
