@@ -182,21 +182,38 @@ __global__ __launch_bounds__(256) void k_hogbom_step(HogbomArgs a, int i)
         a.model[pos].x = a.model[pos].x + peak * a.gain;
         a.keys[(i + 2) % 3] = 0ull;
     }
-    // rows split over blockIdx.y, columns over blockIdx.x: no per-pixel division
+    // rows split over blockIdx.y, columns over blockIdx.x: no per-pixel division.
+    // kRows rows a step, all their loads issued before any use, so that a
+    // thread has that many reads in flight (one at a time left it waiting on
+    // memory latency row after row).
+    constexpr int kRows = 8;
     unsigned long long best = 0;
-    for (int y = blockIdx.y; y < W; y += gridDim.y) {
-        const int sy = y - diffy;
-        const bool rowIn = y >= starty && y <= stopy && sy >= 0 && sy < PW;
-        float *__restrict__ rrow = a.res + (size_t)y * W;
-        const float *__restrict__ prow = rowIn ? a.psf + ((ptrdiff_t)sy * PW - diffx) : a.psf;
-        for (int x = blockIdx.x * 256 + threadIdx.x; x < W; x += gridDim.x * 256) {
-            float v = rrow[x];
-            if (rowIn && x >= startx && x <= stopx) {
-                v = v - amp * prow[x];
-                rrow[x] = v;
+    for (int x = blockIdx.x * 256 + threadIdx.x; x < W; x += gridDim.x * 256) {
+        const bool colIn = x >= startx && x <= stopx;
+        for (int y0 = blockIdx.y; y0 < W; y0 += gridDim.y * kRows) {
+            float v[kRows], pv[kRows];
+            bool sub[kRows];
+#pragma unroll
+            for (int u = 0; u < kRows; ++u) {
+                const int y = y0 + u * (int)gridDim.y;
+                const int sy = y - diffy;
+                const bool rowOk = y < W;
+                sub[u] = rowOk && colIn && y >= starty && y <= stopy && sy >= 0 && sy < PW;
+                v[u] = rowOk ? a.res[(size_t)y * W + x] : 0.f;
+                pv[u] = sub[u] ? a.psf[(ptrdiff_t)sy * PW - diffx + x] : 0.f;
             }
-            const unsigned long long k = peak_key(v, (unsigned)y * (unsigned)W + (unsigned)x);
-            best = k > best ? k : best;
+#pragma unroll
+            for (int u = 0; u < kRows; ++u) {
+                const int y = y0 + u * (int)gridDim.y;
+                if (y >= W) continue;
+                float vv = v[u];
+                if (sub[u]) {
+                    vv = vv - amp * pv[u];
+                    a.res[(size_t)y * W + x] = vv;
+                }
+                const unsigned long long k = peak_key(vv, (unsigned)y * (unsigned)W + (unsigned)x);
+                best = k > best ? k : best;
+            }
         }
     }
     block_peak(best, a.keys + (i + 1) % 3);
