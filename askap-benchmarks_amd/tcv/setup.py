@@ -2,16 +2,18 @@
 C(w) tables (computed by libtcv's host code), plus the synthetic workloads.
 
 Workloads
-  config1  attic tConvolve (attic/tConvolveMPI/Benchmark.h:39-46): 160 000
-           samples x 1 channel from the 64-bit LCG, data 1+0i, 4096^2 grid,
-           wSize 33, overSample 8, cellSize 5 wavelengths, baseline 2000 m
-           (u, v, w in [-1000, 1000] m as the reference generates them); the
-           declared variant 'wide' uses baseline 4000 m, i.e. [-2000, 2000] m.
-  config2  spectral scale: 1 048 576 samples x 8 channels, u, v, w uniform in
-           [-6000, 6000] m (numpy PCG64, seeded), data complex Gaussian with
-           E|d|^2 = 1, freq (1.4e9 + c*1e6)/2.998e8, cellSize 15 wavelengths
-           (support 64 by the reference formula), 4096^2 grid, wSize 129,
-           overSample 8, attic C(w) with baseline 6000 m.
+  attic_problem     attic tConvolve (attic/tConvolveMPI/Benchmark.h:39-46):
+                    160 000 samples x 1 channel from the 64-bit LCG, data 1+0i,
+                    4096^2 grid, wSize 33, overSample 8, cellSize 5 wavelengths,
+                    baseline 2000 m (u, v, w in [-1000, 1000] m as the reference
+                    generates them); the declared variant 'wide' uses baseline
+                    4000 m, i.e. [-2000, 2000] m.
+  spectral_problem  spectral scale: 1 048 576 samples x 8 channels, u, v, w
+                    uniform in [-6000, 6000] m (numpy PCG64, seeded), data complex
+                    Gaussian with E|d|^2 = 1, freq (1.4e9 + c*1e6)/2.998e8, cellSize
+                    15 wavelengths (support 64 by the reference formula), 4096^2
+                    grid, wSize 129, overSample 8, attic C(w) with baseline 6000 m.
+  uneven_batch      many independent jobs of uneven size for batching and serving.
 """
 from __future__ import annotations
 
@@ -140,18 +142,18 @@ class Problem:
         return self.u.size * self.freq.size
 
 
-def config1(variant: str = "reference", nSamples: int = 160000) -> Problem:
+def attic_problem(variant: str = "reference", nSamples: int = 160000) -> Problem:
     baseline = 2000 if variant == "reference" else 4000
     u, v, w = attic_uvw(nSamples, baseline)
     freq = attic_freq(1)
     # the C table and cell sizes always follow the 2000 m reference constants
     support, wcs, C = attic_initC(freq[0], 5.0, 33, 2000, 8)
-    return Problem(f"config1-{variant}", u, v, w, freq, np.ones(nSamples, np.complex64), C,
+    return Problem(f"attic-{variant}", u, v, w, freq, np.ones(nSamples, np.complex64), C,
                    4096, 5.0, wcs, 33, 8, support, np.array([2 * support + 1], np.int32),
                    meta=dict(baseline_m=baseline, seed="LCG next=1"))
 
 
-def config2(nSamples: int = 1 << 20, nChan: int = 8, seed: int = 2024, gSize: int = 4096,
+def spectral_problem(nSamples: int = 1 << 20, nChan: int = 8, seed: int = 2024, gSize: int = 4096,
             wSize: int = 129, cellSize: float = 15.0, baseline: int = 6000) -> Problem:
     rng = np.random.Generator(np.random.PCG64(seed))
     half = baseline
@@ -162,7 +164,7 @@ def config2(nSamples: int = 1 << 20, nChan: int = 8, seed: int = 2024, gSize: in
     n = nSamples * nChan
     data = ((rng.standard_normal(n) + 1j * rng.standard_normal(n)) * np.sqrt(0.5)).astype(np.complex64)
     support, wcs, C = attic_initC(freq[0], cellSize, wSize, baseline, 8)
-    return Problem("config2", u, v, w, freq, data, C, gSize, cellSize, wcs, wSize, 8, support,
+    return Problem("spectral", u, v, w, freq, data, C, gSize, cellSize, wcs, wSize, 8, support,
                    np.array([2 * support + 1], np.int32),
                    meta=dict(seed=seed, baseline_m=baseline, rng="numpy PCG64"))
 
@@ -204,7 +206,7 @@ def tightest_cell(baseline: float, gSize: int, freq0: float = F0) -> float:
     raise ValueError("no cell size fits")
 
 
-def config3(nJobs: int = 64, seed: int = 3, vis_min: float = 1e4, vis_max: float = 2e6,
+def uneven_batch(nJobs: int = 64, seed: int = 3, vis_min: float = 1e4, vis_max: float = 2e6,
             grids=(1024, 2048, 4096), wsizes=(33, 65, 129)):
     """The uneven batch: per job log-uniform visibility count, grid size and
     wSize drawn from the sets, baseline uniform integer in [1000, 6000] m, u, v, w

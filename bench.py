@@ -153,7 +153,7 @@ def bench_batch(ctx, args):
     from tcv import setup as S
     from tcv.gridder import BatchJobs
     t0 = time.time()
-    specs = S.config3(nJobs=args.batch_jobs)
+    specs = S.uneven_batch(nJobs=args.batch_jobs)
     bj = BatchJobs(ctx, specs)
     log(f"batch: {len(specs)} jobs, {sum(s['nVis'] for s in specs)} visibilities, "
         f"{bj.points / 1e9:.1f} G points per pass, set up in {time.time() - t0:.1f} s")
@@ -266,7 +266,7 @@ def serving_setup(ctx):
     within a 200 GB budget."""
     from tcv import setup as S
     from tcv.gridder import BatchJobs
-    specs = S.config3(nJobs=64)
+    specs = S.uneven_batch(nJobs=64)
     bj = BatchJobs(ctx, specs)
     bj.zero_grids()
     bj.run_batched(graph=False)
@@ -351,7 +351,7 @@ def main():
     from tcv import setup as S
 
     t0 = time.time()
-    prob = S.config2(nSamples=args.nsamples, nChan=args.nchan)
+    prob = S.spectral_problem(nSamples=args.nsamples, nChan=args.nchan)
     log(f"problem built in {time.time() - t0:.1f} s: {prob.nVis} visibilities, "
         f"C {prob.C.nbytes / 2**30:.2f} GiB")
     ctx = tcv.Context(0)

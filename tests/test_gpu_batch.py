@@ -17,7 +17,7 @@ TOL = 1e-5
 
 
 def small_specs(n=5, seed=11):
-    return S.config3(nJobs=n, seed=seed, vis_min=2e3, vis_max=2e4, grids=(512, 1024, 2048))
+    return S.uneven_batch(nJobs=n, seed=seed, vis_min=2e3, vis_max=2e4, grids=(512, 1024, 2048))
 
 
 def oracle_job(bj, j, grid_in=None):

@@ -53,7 +53,7 @@ def gpu_degrid(ctx, vis, grid, C, method, tuning=None):
 
 # ---- a6: index set-up, bit-exact ---------------------------------------------
 
-def test_offsets_config1_bitexact_vs_reference(ctx):
+def test_offsets_attic_bitexact_vs_reference(ctx):
     meta, arrs = golden("attic")
     u, v, w = S.attic_uvw(meta["nSamples"], meta["baseline"])
     freq = S.attic_freq(1)
@@ -88,7 +88,7 @@ def test_offsets_current_bitexact_vs_reference(ctx, rt):
 
 
 def test_offsets_spectral_vs_oracle(ctx):
-    prob = S.config2(nSamples=1 << 16)
+    prob = S.spectral_problem(nSamples=1 << 16)
     iu, iv, wp, co, npp, bad = tcv.init_offsets(
         ctx, ctx.to_device(prob.u), ctx.to_device(prob.v), ctx.to_device(prob.w), prob.freq,
         prob.cellSize, prob.wCellSize, prob.wSize, prob.gSize, 8, prob.sSize, None)
@@ -257,7 +257,7 @@ def test_out_of_range_visibilities_are_skipped(ctx):
 # ---- the drop-in Benchmark surfaces ---------------------------------------------
 
 @pytest.mark.parametrize("method", ["tiled", "plain"])
-def test_attic_benchmark_config1_vs_reference(ctx, method):
+def test_attic_benchmark_vs_reference(ctx, method):
     """Full config-1 workload through the drop-in Benchmark; checked against the
     compiled reference's fixture pixels and the oracle's full grid/degrid."""
     meta, arrs = golden("attic")
@@ -322,7 +322,7 @@ def test_current_benchmark_vs_oracle(ctx, rt):
 
 def test_cpp_dropin_driver(ctx):
     """The tConvolveMPI-style C++ driver over the header-only drop-in
-    (include/tcv_benchmark.hpp) runs config 1 on the device; its grid and
+    (include/tcv_benchmark.hpp) runs the attic workload on the device; its grid and
     degrid checksums match the compiled reference's."""
     import os
     import subprocess

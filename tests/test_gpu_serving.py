@@ -20,7 +20,7 @@ GB = 1 << 30
 
 @pytest.fixture(scope="module")
 def templates(ctx):
-    bj = BatchJobs(ctx, S.config3(nJobs=6, seed=21, vis_min=3e3, vis_max=3e4, grids=(512, 1024, 2048)))
+    bj = BatchJobs(ctx, S.uneven_batch(nJobs=6, seed=21, vis_min=3e3, vis_max=3e4, grids=(512, 1024, 2048)))
     bj.zero_grids()
     bj.run_batched(graph=False)     # resident grids for the degrid requests
     ctx.sync()

@@ -100,7 +100,7 @@ def test_current_wavenumber_and_required_rate():
 def test_spectral_workload_fits_the_grid():
     """Every channel of the spectral workload lands inside the grid and the
     w-plane range (checked with the oracle's index set-up on a sample)."""
-    prob = S.config2(nSamples=1 << 15)
+    prob = S.spectral_problem(nSamples=1 << 15)
     iu, iv, co = O.attic_initCOffset(prob.u, prob.v, prob.w, prob.freq, prob.cellSize,
                                      prob.wCellSize, prob.wSize, prob.gSize, prob.support, 8)
     s = int(prob.sSize[0])
