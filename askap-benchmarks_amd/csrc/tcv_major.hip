@@ -161,6 +161,7 @@ struct HogbomArgs {
 // reduces the updated residual's peak into keys[(i + 1) % 3].  keys[(i + 2) % 3]
 // (read by iteration i - 1, written by iteration i + 1) is cleared here.
 // Arithmetic is rounded exactly as the reference's float expressions.
+template <bool VEC>
 __global__ __launch_bounds__(256) void k_hogbom_step(HogbomArgs a, int i)
 {
     // no fused multiply-add: each product and difference is rounded on its own,
@@ -188,6 +189,54 @@ __global__ __launch_bounds__(256) void k_hogbom_step(HogbomArgs a, int i)
     // memory latency row after row).
     constexpr int kRows = 8;
     unsigned long long best = 0;
+    if constexpr (VEC) {
+        // W % 4 == 0: four adjacent pixels a thread, one 16-byte residual load
+        // and store per row (the PSF row is read per pixel: any alignment)
+        for (int x4 = (blockIdx.x * 256 + threadIdx.x) * 4; x4 < W; x4 += gridDim.x * 1024) {
+            for (int y0 = blockIdx.y; y0 < W; y0 += gridDim.y * kRows) {
+                float4 v[kRows];
+                float pv[kRows][4];
+                bool sub[kRows][4];
+#pragma unroll
+                for (int u = 0; u < kRows; ++u) {
+                    const int y = y0 + u * (int)gridDim.y;
+                    const int sy = y - diffy;
+                    const bool rowOk = y < W;
+                    const bool rowSub = rowOk && y >= starty && y <= stopy && sy >= 0 && sy < PW;
+                    v[u] = rowOk ? *reinterpret_cast<const float4 *>(a.res + (size_t)y * W + x4)
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const int x = x4 + c;
+                        sub[u][c] = rowSub && x >= startx && x <= stopx;
+                        pv[u][c] = sub[u][c] ? a.psf[(ptrdiff_t)sy * PW - diffx + x] : 0.f;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kRows; ++u) {
+                    const int y = y0 + u * (int)gridDim.y;
+                    if (y >= W) continue;
+                    float vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+                    bool any = false;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        if (sub[u][c]) {
+                            vv[c] = vv[c] - amp * pv[u][c];
+                            any = true;
+                        }
+                        const unsigned long long k =
+                            peak_key(vv[c], (unsigned)y * (unsigned)W + (unsigned)(x4 + c));
+                        best = k > best ? k : best;
+                    }
+                    if (any)
+                        *reinterpret_cast<float4 *>(a.res + (size_t)y * W + x4) =
+                            make_float4(vv[0], vv[1], vv[2], vv[3]);
+                }
+            }
+        }
+        block_peak(best, a.keys + (i + 1) % 3);
+        return;
+    }
     for (int x = blockIdx.x * 256 + threadIdx.x; x < W; x += gridDim.x * 256) {
         const bool colIn = x >= startx && x <= stopx;
         for (int y0 = blockIdx.y; y0 < W; y0 += gridDim.y * kRows) {
@@ -399,10 +448,17 @@ tcv_status tcv_hogbom(tcv_ctx *ctx, float *d_residual, long long width, const fl
     if (dirtyPeakVal || dirtyPeakPos)
         TCV_HIP(ctx, hipMemcpyAsync(&first, keys, 8, hipMemcpyDeviceToHost, ctx->stream));
     HogbomArgs a{res, psf, (float2 *)d_model, keys, (int)width, (int)psfWidth, gain};
-    const int bx = (int)std::min<long long>((width + 255) / 256, 64);
+    // four pixels a thread when the rows are whole float4s
+    const bool vec = width % 4 == 0;
+    const long long perRow = vec ? width / 4 : width;
+    const int bx = (int)std::min<long long>((perRow + 255) / 256, 64);
     const int by = (int)std::max<long long>(1, std::min<long long>(width, (long long)ctx->cus * 8 / bx));
-    for (int i = 0; i < niters; ++i)
-        hipLaunchKernelGGL(k_hogbom_step, dim3(bx, by), dim3(256), 0, ctx->stream, a, i);
+    for (int i = 0; i < niters; ++i) {
+        if (vec)
+            hipLaunchKernelGGL(k_hogbom_step<true>, dim3(bx, by), dim3(256), 0, ctx->stream, a, i);
+        else
+            hipLaunchKernelGGL(k_hogbom_step<false>, dim3(bx, by), dim3(256), 0, ctx->stream, a, i);
+    }
     TCV_HIP(ctx, hipGetLastError());
     hipLaunchKernelGGL(k_real_writeback, dim3(stream_blocks(ctx, n, 4)), dim3(256), 0, ctx->stream,
                        (float2 *)d_residual, (const float *)res, n);

@@ -99,12 +99,13 @@ def test_hogbom_bitexact(ctx, g, niters):
     assert np.array_equal(d_model.download(), m_ref)
 
 
-def test_hogbom_small_psf_and_off_diagonal_peak(ctx):
+@pytest.mark.parametrize("g,pw", [(96, 40), (101, 37)])
+def test_hogbom_small_psf_and_off_diagonal_peak(ctx, g, pw):
     """PSF narrower than the image, with its peak off the diagonal: the
     reference's row offset (ry - px) is reproduced, rows it would read outside
-    the PSF are skipped, exactly as the oracle does."""
-    rng = np.random.default_rng(12)
-    g, pw = 96, 40
+    the PSF are skipped, exactly as the oracle does.  Widths that are a
+    multiple of four take the four-pixel path, others the one-pixel path."""
+    rng = np.random.default_rng(12 + g)
     dirty = _img(rng, g, real=True)
     psf = (0.1 * _img(rng, pw, real=True)).astype(np.complex64)
     psf[9, 23] = 5.0                                   # peak at px=23, py=9
