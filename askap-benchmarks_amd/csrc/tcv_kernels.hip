@@ -800,19 +800,19 @@ __device__ __forceinline__ int first_item(const ItemSource<BATCH> &src)
 // cells outside the grid as zero.  Every thread issues all its loads, two
 // cells (16 bytes) each, before its first LDS write, so that their latencies
 // overlap instead of adding up.
-template <int TW, int TH>
+template <int TW, int TH, int NT = 256>
 __device__ __forceinline__ void load_tile(float2 *tile, const float2 *__restrict__ grid, int gSize,
                                           int Bx, int By)
 {
     constexpr int kPairs = TH * TW / 2;
-    constexpr int kPer = (kPairs + 255) / 256;
+    constexpr int kPer = (kPairs + NT - 1) / NT;
     constexpr int kBlk = kPer < 8 ? kPer : 8;   // loads in flight per thread (register bound)
 #pragma unroll
     for (int i0 = 0; i0 < kPer; i0 += kBlk) {
         float4 v[kBlk];
 #pragma unroll
         for (int b = 0; b < kBlk; ++b) {
-            const int q = 2 * ((int)threadIdx.x + 256 * (i0 + b));   // first cell of the pair
+            const int q = 2 * ((int)threadIdx.x + NT * (i0 + b));   // first cell of the pair
             const int row = By + q / TW, col = Bx + q % TW;
             v[b] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (i0 + b < kPer && q < TH * TW && row < gSize) {
@@ -828,7 +828,7 @@ __device__ __forceinline__ void load_tile(float2 *tile, const float2 *__restrict
         }
 #pragma unroll
         for (int b = 0; b < kBlk; ++b) {
-            const int q = 2 * ((int)threadIdx.x + 256 * (i0 + b));
+            const int q = 2 * ((int)threadIdx.x + NT * (i0 + b));
             if (i0 + b < kPer && q < TH * TW) *reinterpret_cast<float4 *>(tile + q) = v[b];
         }
         // keep the next block's loads after these stores
@@ -1559,9 +1559,17 @@ __global__ __launch_bounds__(256) void k_degrid_lds(ItemSource<BATCH> src)
 // outside the table, the last column gets one pass with a lane per row
 // instead.  Index arithmetic is 24-bit (full-rate), and only the last step of
 // an entry checks for lanes past its end.
+#ifndef TCV_WIDE_DEGRID_WAVES
+#define TCV_WIDE_DEGRID_WAVES 8
+#endif
+// waves of the pair degridder: 64 KB tiles (two workgroups a CU) get more
+template <int TW, int TH>
+constexpr int degrid2_waves() { return TW * TH >= 8192 ? TCV_WIDE_DEGRID_WAVES : 4; }
+
 template <int TW, int TH, bool HALF, bool BATCH>
-__global__ __launch_bounds__(256) void k_degrid_lds2(ItemSource<BATCH> src)
+__global__ __launch_bounds__((degrid2_waves<TW, TH>() * 64)) void k_degrid_lds2(ItemSource<BATCH> src)
 {
+    constexpr int NWD = degrid2_waves<TW, TH>();
     constexpr unsigned TB = Taps<HALF>::kBytes;
     constexpr unsigned kLastPair = (TH * TW - 2) * 8u;   // byte offset of the tile's last pair
     __shared__ float2 tile[TH * TW];
@@ -1586,11 +1594,11 @@ __global__ __launch_bounds__(256) void k_degrid_lds2(ItemSource<BATCH> src)
         const int4 it = cload(jd.items, local);
         const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
         const int Bx = txi * TW, By = tyi * TH;
-        load_tile<TW, TH>(tile, grid, gSize, Bx, By);
+        load_tile<TW, TH, NWD * 64>(tile, grid, gSize, Bx, By);
         __syncthreads();
 
         const int eEnd = it.y + it.z;
-        for (int e = it.y + wave; e < eEnd; e += 4) {
+        for (int e = it.y + wave; e < eEnd; e += NWD) {
             const int4 en = cload(entries, e);
             const int u0 = en.x & 0xFFFF, v0 = (int)((unsigned)en.x >> 16);
             const int c0 = en.y, s = entry_width(en.w);
@@ -2180,12 +2188,17 @@ tcv_status degrid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_k
         else if (kern_kind < 3)
             kern = k_degrid_tiled<TW, T::RD, kNW, (T::RD % kRB == 0 ? kRB : T::RD), HALF, BATCH>;
     }
+    int threads = kNW * 64;
+    if (kern == k_degrid_lds2<TW, TH, HALF, BATCH>) threads = degrid2_waves<TW, TH>() * 64;
     if constexpr (HALF && !T::kPairOnly) {
-        if (kern_kind == 4) kern = k_degrid_lds4h<TW, TH, BATCH>;   // four FP16 taps per load
+        if (kern_kind == 4) {
+            kern = k_degrid_lds4h<TW, TH, BATCH>;   // four FP16 taps per load
+            threads = kNW * 64;
+        }
     }
-    const int blocks = persistent_blocks(ctx, (const void *)kern, kNW * 64);
+    const int blocks = persistent_blocks(ctx, (const void *)kern, threads);
     ProfScope ps(ctx, BATCH ? "degrid_batch" : (HALF ? "degrid_tiled_half" : "degrid_tiled"));
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kNW * 64), 0, ctx->stream, src);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), 0, ctx->stream, src);
     TCV_HIP(ctx, hipGetLastError());
     return TCV_OK;
 }
