@@ -464,21 +464,22 @@ constexpr double kGridPhaseBytes = 5.5e6;   // the gridder may use its own count
 // 128 x 64 tiles carry half again as many entries per work item: smaller phases
 // (measured optimum 256-320 phases on a 1.1 GB table, against 184-216 for 64 x 64)
 constexpr double kWidePhaseBytes = 3.8e6;
+constexpr int kWideTileMinWidth = 89;
 
 static void auto_tuning(int maxS, const tcv_geometry *g, tcv_tuning *t)
 {
     int minS = maxS;
     for (int k = 0; k < g->nPlanes; k++) minS = std::min(minS, g->sSize[k]);
-    // 128-wide tiles (pair kernels only) for kernels wider than two 64-column
-    // tiles: each footprint row then meets two tiles instead of three and each
-    // visibility makes a third fewer entries.  Measured 9 % faster at width 129,
-    // 2-5 % slower at widths 113 and 121, where 64-wide tiles are kept.
-    // (FP16 storage: 64-wide tiles measured 18 % faster than 128-wide at width 129)
+    // 128-wide tiles (pair kernels only) for kernels 89 wide and up: a footprint
+    // row meets fewer tiles, each visibility makes fewer entries, and the
+    // degridder's eight-wave workgroups walk them.  Grid + degrid measured 20 %
+    // faster at width 129, 10-11 % at 113 and 121, 6 % at 97, equal at 81 and
+    // 4 % slower at 65.  (FP16 storage keeps 64-wide tiles.)
     if (t->tap_bytes <= 0) t->tap_bytes = 8;
     const bool pairs = minS >= 2 && (t->kernel <= 0 || t->kernel == 3) &&
                        (t->degrid_kernel <= 0 || t->degrid_kernel == 3);
     if (t->tile_w <= 0)
-        t->tile_w = (maxS > 128 && pairs && t->tile_h <= 64 && t->tap_bytes == 8) ? 128
+        t->tile_w = (maxS >= kWideTileMinWidth && pairs && t->tile_h <= 64 && t->tap_bytes == 8) ? 128
                     : maxS >= 48 ? 64 : maxS >= 24 ? 32 : maxS >= 12 ? 16 : 8;
     if (t->tile_h <= 0) t->tile_h = 64;
     if (t->chunk <= 0) t->chunk = 2048;
@@ -982,7 +983,7 @@ std::string tuning_class(const tcv_geometry *g)
 // Tile shapes worth trying for the widest kernel of a job.
 static std::vector<std::pair<int, int>> candidate_tiles(int maxS)
 {
-    if (maxS >= 96) return {{64, 64}, {128, 64}, {64, 32}};
+    if (maxS >= 64) return {{128, 64}, {64, 64}, {64, 32}};
     if (maxS >= 48) return {{64, 64}, {64, 32}, {32, 64}};
     if (maxS >= 24) return {{32, 64}, {32, 32}, {64, 32}};
     if (maxS >= 12) return {{16, 64}, {16, 32}, {32, 32}};

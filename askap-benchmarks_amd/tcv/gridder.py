@@ -103,15 +103,18 @@ class GridJob:
 __all__ = ["init_offsets", "GridJob", "TcvError", "batch_tile_w"]
 
 
+WIDE_TILE_MIN_WIDTH = 89   # the library's rule for a single plan (FP32 tables)
+
+
 def batch_tile_w(widths, tuning: dict | None = None) -> int:
     """The one tile width shared by a batch (or a server's requests): 128 when
-    the widest kernel is wider than 128 (the rule a single job's plan follows),
-    every kernel is at least 2 wide and the pair kernels (code 3, or automatic)
-    are used; else 64."""
+    the widest kernel is at least 89 wide (the rule a single job's plan
+    follows), every kernel is at least 2 wide and the pair kernels (code 3, or
+    automatic) are used; else 64."""
     t = tuning or {}
     pairs = t.get("kernel", 0) in (0, 3) and t.get("degrid_kernel", 0) in (0, 3)
     widths = [int(w) for w in widths]
-    return 128 if pairs and max(widths) > 128 and min(widths) >= 2 else 64
+    return 128 if pairs and max(widths) >= WIDE_TILE_MIN_WIDTH and min(widths) >= 2 else 64
 
 
 class BatchJobs:

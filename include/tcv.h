@@ -193,7 +193,7 @@ tcv_status tcv_range_errors(tcv_ctx *ctx, long long *count);
 
 typedef struct {
     int tile_w;      /* 8, 16, 32, 64 or 128 columns (0 = auto: 128 x 64 tiles for
-                        kernels wider than 128 when the pair kernels apply); 128-wide
+                        kernels 89 wide and up when the pair kernels apply); 128-wide
                         tiles take only the pair kernels (kernel and degrid_kernel 3) */
     int tile_h;      /* rows per tile: 32 or 64 (64 for 128-wide tiles; 0 = auto) */
     int chunk;       /* max entries per work item (0 = auto) */

@@ -160,8 +160,8 @@ def test_tiled_shapes_vs_oracle(ctx, tw, th):
 
 @pytest.mark.parametrize("Sz,gSize", [(129, 300), (97, 257), (5, 200)])
 def test_wide_tiles_pair_kernels_vs_oracle(ctx, Sz, gSize):
-    """128 x 64 tiles (pair kernels only; the default for kernels wider than
-    128): grid and degrid within 1e-5 of the oracle over item splits,
+    """128 x 64 tiles (pair kernels only; the default for kernels 89 wide and
+    up): grid and degrid within 1e-5 of the oracle over item splits,
     phase counts and the XCD queues, on grids that are not a multiple of the
     tile (257, 300) and with footprints narrower than the tile."""
     iu, iv, co, data, C = small_problem(160 + Sz, 1500, gSize, Sz, cluster=60)
@@ -183,7 +183,7 @@ def test_wide_tiles_pair_kernels_vs_oracle(ctx, Sz, gSize):
 
 
 def test_wide_tiles_default_and_rejections(ctx):
-    """Kernels wider than 128 get 128 x 64 tiles by default, narrower ones
+    """Kernels 89 wide and up get 128 x 64 tiles by default, narrower ones
     64-wide tiles; 128-wide tiles with a one-column kernel (1, 2) or with
     width-1 kernels are refused with an error."""
     iu, iv, co, data, C = small_problem(170, 200, 400, 129)
@@ -194,8 +194,11 @@ def test_wide_tiles_default_and_rejections(ctx):
     for bad in (dict(tile_w=128, tile_h=64, kernel=1), dict(tile_w=128, tile_h=64, degrid_kernel=2)):
         with pytest.raises(tcv.TcvError):
             tcv.Plan(ctx, vis, **bad)
-    iu, iv, co, data, C = small_problem(171, 200, 256, 121)
-    vis = vis_from_host(ctx, iu, iv, co, [121], 256, C.size)
+    iu, iv, co, data, C = small_problem(171, 200, 256, 89)
+    vis = vis_from_host(ctx, iu, iv, co, [89], 256, C.size)
+    assert tcv.Plan(ctx, vis).info()["tile_w"] == 128
+    iu, iv, co, data, C = small_problem(173, 200, 256, 87)
+    vis = vis_from_host(ctx, iu, iv, co, [87], 256, C.size)
     assert tcv.Plan(ctx, vis).info()["tile_w"] == 64
     iu, iv, co, data, C = small_problem(172, 200, 64, 1)
     vis = vis_from_host(ctx, iu, iv, co, [1], 64, C.size)
