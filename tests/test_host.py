@@ -134,3 +134,15 @@ def test_ctypes_structs_match_the_header(tmp_path):
     got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True,
                                           text=True).stdout.split()]
     assert got == want
+
+
+def test_batch_tile_w_rule():
+    """One tile width for a batch: 128 from width 89 with the pair kernels and
+    no width-1 kernel, else 64 (the single-plan rule applied to the widest)."""
+    from tcv.gridder import batch_tile_w
+    assert batch_tile_w([63, 91, 131]) == 128
+    assert batch_tile_w([89]) == 128 and batch_tile_w([87]) == 64
+    assert batch_tile_w([1, 131]) == 64
+    assert batch_tile_w([131], dict(kernel=1)) == 64
+    assert batch_tile_w([131], dict(degrid_kernel=2)) == 64
+    assert batch_tile_w([131], dict(kernel=3, degrid_kernel=3)) == 128
