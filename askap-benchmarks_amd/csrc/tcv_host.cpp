@@ -413,6 +413,37 @@ tcv_status tcv_degrid_plain(tcv_ctx *ctx, const tcv_geometry *g, const float *d_
     return st;
 }
 
+tcv_status tcv_grid_plain_f64(tcv_ctx *ctx, const tcv_geometry *g, const float *d_data,
+                              const float *d_C, double *d_grid)
+{
+    int maxS = 0;
+    tcv_status st = check_geometry(ctx, g, &maxS);
+    if (st != TCV_OK) return st;
+    if (g->nVis > 0 && (!d_C || !d_grid)) return tcv_fail(ctx, TCV_ERR_INVALID, "null C or grid");
+    GeomUpload up;
+    GeomDev gd;
+    if ((st = make_geom_dev(ctx, g, up, gd)) != TCV_OK) return st;
+    st = launch_grid_plain_f64(ctx, gd, (const float2 *)d_data, (const float2 *)d_C, (double2 *)d_grid);
+    if (up.d_s) TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return st;
+}
+
+tcv_status tcv_degrid_plain_f64(tcv_ctx *ctx, const tcv_geometry *g, const float *d_grid,
+                                const float *d_C, double *d_data)
+{
+    int maxS = 0;
+    tcv_status st = check_geometry(ctx, g, &maxS);
+    if (st != TCV_OK) return st;
+    if (g->nVis > 0 && (!d_data || !d_C || !d_grid))
+        return tcv_fail(ctx, TCV_ERR_INVALID, "null data, C or grid");
+    GeomUpload up;
+    GeomDev gd;
+    if ((st = make_geom_dev(ctx, g, up, gd)) != TCV_OK) return st;
+    st = launch_degrid_plain_f64(ctx, gd, (const float2 *)d_grid, (const float2 *)d_C, (double2 *)d_data);
+    if (up.d_s) TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return st;
+}
+
 tcv_status tcv_range_errors(tcv_ctx *ctx, long long *count)
 {
     if (!ctx || !count) return TCV_ERR_INVALID;
@@ -625,6 +656,23 @@ void plan_bind(tcv_plan *p, char *block)
         p->metaG = p->meta;
     }
     p->block = block;
+}
+
+void mark_shared_grids(std::vector<JobDesc> &jobs)
+{
+    // grid byte ranges [grid, grid + 8 gSize^2); every pair compared (batches
+    // hold tens to hundreds of jobs)
+    const size_t n = jobs.size();
+    std::vector<uintptr_t> b(n), e(n);
+    for (size_t j = 0; j < n; ++j) {
+        jobs[j].sharedGrid = 0;
+        b[j] = reinterpret_cast<uintptr_t>(jobs[j].grid);
+        e[j] = b[j] + sizeof(float2) * (uintptr_t)jobs[j].gSize * (uintptr_t)jobs[j].gSize;
+    }
+    for (size_t j = 0; j < n; ++j)
+        for (size_t k = j + 1; k < n; ++k)
+            if (jobs[j].grid && jobs[k].grid && b[j] < e[k] && b[k] < e[j])
+                jobs[j].sharedGrid = jobs[k].sharedGrid = 1;
 }
 
 extern "C" {
@@ -847,6 +895,11 @@ tcv_status tcv_batch_create(tcv_ctx *ctx, int nJobs, const tcv_job *jobs, tcv_ba
         b->host.push_back(make_job_desc(J.plan, true, J.d_data, J.d_C, (float2 *)J.d_grid, (float2 *)J.d_out, nullptr));
         b->hostD.push_back(make_job_desc(J.plan, false, J.d_data, J.d_C, (float2 *)J.d_grid, (float2 *)J.d_out, nullptr));
     }
+    // Jobs whose grids overlap (several channels into one grid, say) run in one
+    // persistent launch: a work item that is the only one of its tile within
+    // its own job may still meet another job's item on the same cells, so those
+    // jobs flush every tile with atomics.
+    mark_shared_grids(b->host);
     auto fail = [&](const char *m) {
         if (b->d_jobs) (void)hipFree(b->d_jobs);
         if (b->d_jobsD) (void)hipFree(b->d_jobsD);

@@ -84,6 +84,10 @@ tcv_status launch_grid_plain(tcv_ctx *ctx, const GeomDev &g, int maxS,
                              const float2 *data, const float2 *C, float2 *grid);
 tcv_status launch_degrid_plain(tcv_ctx *ctx, const GeomDev &g,
                                const float2 *grid, const float2 *C, float2 *out);
+tcv_status launch_grid_plain_f64(tcv_ctx *ctx, const GeomDev &g, const float2 *data, const float2 *C,
+                                 double2 *grid);
+tcv_status launch_degrid_plain_f64(tcv_ctx *ctx, const GeomDev &g, const float2 *grid, const float2 *C,
+                                   double2 *out);
 
 // ---- tiled path -------------------------------------------------------------
 
@@ -108,6 +112,9 @@ struct JobDesc {
     // entry's is above cEdgeHi, may hold entries on the table's first or last
     // slice (entries are ordered by C-slice key within an item)
     int cEdgeLo, cEdgeHi;
+    // 1 when another job of the same launch grids into an overlapping grid: a
+    // work item that holds a tile alone then still flushes with atomics
+    int sharedGrid;
 };
 
 constexpr int kMaxPersistentBlocks = 8192;
@@ -193,6 +200,8 @@ struct HalfStorage {
 };
 
 tcv_status plan_build_kernels(tcv_ctx *ctx, tcv_plan *p, const GeomDev &g);
+// Sets JobDesc::sharedGrid on every job whose grid overlaps another job's.
+void mark_shared_grids(std::vector<JobDesc> &jobs);
 // half == nullptr: FP32 taps and data
 JobDesc make_job_desc(const tcv_plan *p, bool forGrid, const void *data, const void *C, float2 *grid,
                       float2 *out, const HalfStorage *half);

@@ -320,6 +320,87 @@ __global__ __launch_bounds__(256) void k_degrid_plain(GeomDev g, const float2 *_
     if (lane == 0) out[d] = make_float2(re, im);
 }
 
+// ---- FP64 reference mode of the plain kernels -----------------------------------
+// The same footprint walks as k_grid_plain / k_degrid_plain on the same stored
+// complex64 taps and visibilities, promoted to double: each tap product is
+// exact in FP64 (24-bit x 24-bit significands) and sums accumulate in FP64
+// (grid: FP64 atomics into a complex128 grid; degrid: per-lane FP64 sums and a
+// wave reduction).  Every FP32 or FP16 result can then state its error against
+// an FP64 gridding on the device, at any size.
+__global__ __launch_bounds__(256) void k_grid_plain_f64(GeomDev g, const float2 *__restrict__ data,
+                                                        const float2 *__restrict__ C,
+                                                        double2 *__restrict__ grid,
+                                                        unsigned long long *bad)
+{
+    constexpr int BS = 256;
+    const int d = blockIdx.x;
+    const VisGeom vg = vis_geom(g, d);
+    if (!vg.ok) {
+        if (threadIdx.x == 0) atomicAdd(bad, 1ull);
+        return;
+    }
+    const int s = vg.s;
+    const float2 dv = data ? data[d] : make_float2(1.f, 0.f);
+    const double dr = dv.x, di = dv.y;
+    const int n = s * s;
+    int r = threadIdx.x / s, c = threadIdx.x - (threadIdx.x / s) * s;
+    const int sr = BS / s, sc = BS - (BS / s) * s;
+    const float2 *cp = C + vg.c0;
+    for (int p = threadIdx.x; p < n; p += BS) {
+        const float2 cv = cp[p];
+        const double cr = cv.x, ci = cv.y;
+        double *gp = reinterpret_cast<double *>(grid + ((size_t)(vg.v0 + r) * g.gSize + vg.u0 + c));
+        unsafeAtomicAdd(gp, dr * cr - di * ci);
+        unsafeAtomicAdd(gp + 1, dr * ci + di * cr);
+        c += sc;
+        r += sr;
+        if (c >= s) { c -= s; r++; }
+    }
+}
+
+__device__ __forceinline__ double wave_sum_f64(double x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+__global__ __launch_bounds__(256) void k_degrid_plain_f64(GeomDev g, const float2 *__restrict__ grid,
+                                                          const float2 *__restrict__ C,
+                                                          double2 *__restrict__ out,
+                                                          unsigned long long *bad)
+{
+    const int lane = threadIdx.x & 63;
+    const int d = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (d >= g.nVis) return;   // whole wave leaves together
+    const VisGeom vg = vis_geom(g, d);
+    if (!vg.ok) {
+        if (lane == 0) {
+            atomicAdd(bad, 1ull);
+            out[d] = make_double2(0.0, 0.0);
+        }
+        return;
+    }
+    const int s = vg.s;
+    const int n = s * s;
+    int r = lane / s, c = lane - (lane / s) * s;
+    const int sr = 64 / s, sc = 64 - (64 / s) * s;
+    const float2 *cp = C + vg.c0;
+    double re = 0.0, im = 0.0;
+    for (int p = lane; p < n; p += 64) {
+        const float2 cv = cp[p];
+        const float2 gv = grid[(size_t)(vg.v0 + r) * g.gSize + vg.u0 + c];
+        re += (double)gv.x * cv.x - (double)gv.y * cv.y;
+        im += (double)gv.x * cv.y + (double)gv.y * cv.x;
+        c += sc;
+        r += sr;
+        if (c >= s) { c -= s; r++; }
+    }
+    re = wave_sum_f64(re);
+    im = wave_sum_f64(im);
+    if (lane == 0) out[d] = make_double2(re, im);
+}
+
 // ---- scan (exclusive, int32, total written at out[n]) -------------------------
 
 constexpr int SCAN_BS = 256;
@@ -923,7 +1004,7 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled(ItemSource<BATCH> src)
         const bool colOk = col < gSize;
         float2 *__restrict__ g0 = jd.grid + (size_t)(By + q0 * RPI + ly) * gSize + col;
         const int rowsLeft = gSize - (By + q0 * RPI + ly);   // rows j*RPI < rowsLeft exist
-        if (it.w) {
+        if (it.w && !jd.sharedGrid) {
             // the tile is this item's alone: plain read-modify-write, every read
             // issued before the first write so that their latencies overlap
             float2 o[R];
@@ -1185,7 +1266,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
         const int rowsLeft = gSize - (By + q0 * RPI + ly);
         float2 *__restrict__ g0 = jd.grid + (size_t)(By + q0 * RPI + ly) * gSize + col;
         const bool c0ok = col < gSize, c1ok = col + 1 < gSize;
-        if (it.w) {
+        if (it.w && !jd.sharedGrid) {
             // the tile is this item's alone: plain read-modify-write, every read
             // issued before the first write so that their latencies overlap
             // (RB rows at a time: the old values' registers)
@@ -1339,7 +1420,7 @@ __global__ __launch_bounds__(NW * 64) void k_grid_tiled4h(ItemSource<BATCH> src)
         const int col = Bx + lx;
         const int rowsLeft = gSize - (By + q0 * RPI + ly);
         float2 *__restrict__ g0 = jd.grid + (size_t)(By + q0 * RPI + ly) * gSize + col;
-        if (it.w) {
+        if (it.w && !jd.sharedGrid) {
 #pragma unroll
             for (int j = 0; j < R; ++j)
 #pragma unroll
@@ -1461,7 +1542,7 @@ __global__ __launch_bounds__(256) void k_grid_lds(ItemSource<BATCH> src)
             const float2 v = tile[r * TW + cc];
             if (row < gSize && col < gSize && (v.x != 0.f || v.y != 0.f)) {
                 float2 *gp = jd.grid + (size_t)row * gSize + col;
-                if (it.w) {
+                if (it.w && !jd.sharedGrid) {
                     float2 o = *gp;
                     o.x += v.x;
                     o.y += v.y;
@@ -1967,6 +2048,28 @@ tcv_status launch_degrid_plain(tcv_ctx *ctx, const GeomDev &g, const float2 *gri
     ProfScope ps(ctx, "degrid_plain");
     hipLaunchKernelGGL(k_degrid_plain, dim3((g.nVis + 3) / 4), dim3(256), 0, ctx->stream, g,
                        grid, C, out, ctx->d_counters);
+    TCV_HIP(ctx, hipGetLastError());
+    return TCV_OK;
+}
+
+tcv_status launch_grid_plain_f64(tcv_ctx *ctx, const GeomDev &g, const float2 *data, const float2 *C,
+                                 double2 *grid)
+{
+    if (g.nVis == 0) return TCV_OK;
+    ProfScope ps(ctx, "grid_f64");
+    hipLaunchKernelGGL(k_grid_plain_f64, dim3(g.nVis), dim3(256), 0, ctx->stream, g, data, C, grid,
+                       ctx->d_counters);
+    TCV_HIP(ctx, hipGetLastError());
+    return TCV_OK;
+}
+
+tcv_status launch_degrid_plain_f64(tcv_ctx *ctx, const GeomDev &g, const float2 *grid, const float2 *C,
+                                   double2 *out)
+{
+    if (g.nVis == 0) return TCV_OK;
+    ProfScope ps(ctx, "degrid_f64");
+    hipLaunchKernelGGL(k_degrid_plain_f64, dim3((g.nVis + 3) / 4), dim3(256), 0, ctx->stream, g, grid,
+                       C, out, ctx->d_counters);
     TCV_HIP(ctx, hipGetLastError());
     return TCV_OK;
 }

@@ -215,6 +215,26 @@ def degrid_plain(ctx: Context, vis: Visibilities, grid: DeviceArray, C: DeviceAr
               "degrid_plain")
 
 
+def grid_plain_f64(ctx: Context, vis: Visibilities, data: DeviceArray | None, C: DeviceArray,
+                   grid: DeviceArray):
+    """FP64 reference gridding into a complex128 grid (data None: the PSF)."""
+    if grid.dtype != np.complex128:
+        raise TypeError("grid_plain_f64 accumulates into a complex128 grid")
+    g = vis.geometry()
+    ctx.check(lib().tcv_grid_plain_f64(ctx.h, ct.byref(g), None if data is None else data.ptr, C.ptr,
+                                       grid.ptr), "grid_plain_f64")
+
+
+def degrid_plain_f64(ctx: Context, vis: Visibilities, grid: DeviceArray, C: DeviceArray,
+                     out: DeviceArray):
+    """FP64 reference degridding of a complex64 grid into complex128 outputs."""
+    if out.dtype != np.complex128 or grid.dtype != np.complex64:
+        raise TypeError("degrid_plain_f64 reads a complex64 grid into complex128 outputs")
+    g = vis.geometry()
+    ctx.check(lib().tcv_degrid_plain_f64(ctx.h, ct.byref(g), grid.ptr, C.ptr, out.ptr),
+              "degrid_plain_f64")
+
+
 def autotune(ctx: Context, vis: Visibilities, data: DeviceArray, C: DeviceArray,
              grid: DeviceArray, out: DeviceArray, op: int = 3, candidates=None, reps: int = 2):
     """Times candidate tunings of the tiled path on this job and returns the

@@ -185,6 +185,20 @@ tcv_status tcv_grid_plain(tcv_ctx *ctx, const tcv_geometry *g,
 tcv_status tcv_degrid_plain(tcv_ctx *ctx, const tcv_geometry *g,
                             const float *d_grid, const float *d_C, float *d_data);
 
+/* FP64 reference mode of the two kernels above (no counterpart in the
+ * reference, which states no FP64 result; the extension every FP32 or FP16
+ * result is measured against).  The same footprints over the same stored
+ * complex64 taps and visibilities, promoted to double: every tap product is
+ * exact and sums accumulate in FP64.  Gridding accumulates into a complex128
+ * grid (interleaved double pairs) with FP64 atomics; d_data == NULL grids the
+ * point-spread function (every visibility 1 + 0i).  Degridding reads a
+ * complex64 grid and overwrites complex128 outputs.  Out-of-range
+ * visibilities are skipped and counted as by the FP32 kernels. */
+tcv_status tcv_grid_plain_f64(tcv_ctx *ctx, const tcv_geometry *g, const float *d_data,
+                              const float *d_C, double *d_grid);
+tcv_status tcv_degrid_plain_f64(tcv_ctx *ctx, const tcv_geometry *g, const float *d_grid,
+                                const float *d_C, double *d_data);
+
 /* Waits and returns (then clears) the number of visibilities skipped by the
  * kernels above because their footprint or slice was out of range. */
 tcv_status tcv_range_errors(tcv_ctx *ctx, long long *count);

@@ -68,3 +68,16 @@ def small_problem(seed, nVis, gSize, S, os_=8, nPlanes=5, complex_=True, margin=
             np.sqrt(0.5)).astype(np.complex64)
     C = random_C(rng, S, os_, nPlanes, complex_)
     return iu.astype(np.int32), iv.astype(np.int32), cOffset, data, C
+
+
+def exact_problem(seed, nVis, gSize, Sz, cluster):
+    """Inputs whose every partial sum is exact in FP32: C taps are multiples
+    of 1/4 in [-1, 1], data and grid values small integers.  Results then do
+    not depend on summation order, so different kernels, plans and the oracle
+    must agree bit for bit."""
+    iu, iv, co, _, C = small_problem(seed, nVis, gSize, Sz, cluster=cluster)
+    rng = np.random.default_rng(seed + 1)
+    C = (rng.integers(-4, 5, C.size) / 4 + 1j * rng.integers(-4, 5, C.size) / 4).astype(np.complex64)
+    data = (rng.integers(-3, 4, nVis) + 1j * rng.integers(-3, 4, nVis)).astype(np.complex64)
+    grid = (rng.integers(-2, 3, (gSize, gSize)) + 1j * rng.integers(-2, 3, (gSize, gSize))).astype(np.complex64)
+    return iu, iv, co, data, C, grid

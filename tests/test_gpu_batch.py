@@ -113,3 +113,39 @@ def test_batch_other_kernels_vs_oracle(ctx, kernels):
         assert rel_err(g, gref) <= TOL, j
         _, oref = oracle_job(bj, j, grid_in=g)
         assert rel_err(bj.jobs[j]["out"].download(), oref) <= TOL, j
+
+
+@pytest.mark.parametrize("tile_w", [64, 128])
+def test_batch_jobs_sharing_one_grid(ctx, tile_w):
+    """Two jobs gridding into ONE grid in one persistent launch (two channels
+    into one image).  Each job's plan has one C-slice phase and work items
+    larger than any tile's entry list, so within its own job every item holds
+    its tile alone; the batch must still flush those tiles with atomics, since
+    the other job's item for the same tile runs at the same time.  Exact
+    inputs: the result equals the oracle's sum of both jobs bit for bit (a
+    lost read-modify-write would show as a whole missing footprint piece)."""
+    Sz, gSize = 129, 512
+    jobs = [exact_problem(seed, 600, gSize, Sz, cluster=None) for seed in (31, 32)]
+    C = jobs[0][4]
+    d_C = ctx.to_device(C)
+    grid = ctx.zeros((gSize, gSize), np.complex64)
+    keep = []
+    ref = np.zeros((gSize, gSize), np.complex64)
+    for iu, iv, co, data, _, _ in jobs:
+        vis = tcv.Visibilities(ctx, iu, iv, co, [Sz], gSize, C.size)
+        plan = tcv.Plan(ctx, vis, tile_w=tile_w, tile_h=64, phases=1, grid_phases=1,
+                        slice_keys=1, chunk=1 << 20)
+        info = plan.info()
+        # one work item per non-empty tile: every item of the job is exclusive
+        assert info["items"] == len({(u // tile_w, v // 64)
+                                     for u0, v0 in zip(iu - Sz // 2, iv)
+                                     for u in range(u0 // tile_w * tile_w, u0 + Sz, tile_w)
+                                     for v in range(v0 // 64 * 64, v0 + Sz, 64)})
+        keep.append((vis, plan, ctx.to_device(data)))
+        ref = O.grid(data, iu, iv, co, C, gSize, Sz, out=ref)
+    b = tcv.Batch(ctx, [(plan, d, d_C, grid, None) for vis, plan, d in keep])
+    for rep in range(3):
+        grid.zero()
+        b.grid()
+        ctx.sync()
+        assert np.array_equal(grid.download(), ref), rep

@@ -11,7 +11,7 @@ import pytest
 
 import oracle as O
 import tcv
-from helpers import golden, rel_err, small_problem
+from helpers import exact_problem, golden, rel_err, small_problem
 from tcv import setup as S
 
 pytestmark = pytest.mark.gpu
@@ -232,6 +232,32 @@ def test_adjointness_fp64(ctx):
         assert abs(lhs - rhs) <= 1e-5 * max(abs(lhs), 1.0)
 
 
+@pytest.mark.parametrize("Sz", [129, 33])
+def test_fp64_reference_kernels_vs_oracle(ctx, Sz):
+    """The device's FP64 reference mode equals the oracle's FP64 gridding and
+    degridding (same float-stored taps and data, FP64 products and sums) up to
+    FP64 summation order; PSF mode grids data = 1 + 0i."""
+    gSize, n = 384, 2000
+    iu, iv, co, data, C = small_problem(41 + Sz, n, gSize, Sz)
+    vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
+    d_data, d_C = ctx.to_device(data), ctx.to_device(C)
+    g = ctx.zeros((gSize, gSize), np.complex128)
+    tcv.grid_plain_f64(ctx, vis, d_data, d_C, g)
+    ref = O.grid_f64(data, iu, iv, co, C, gSize, Sz)
+    assert rel_err(g.download(), ref) <= 1e-12
+    # a dropped cross term (im = dr*ci only) would be off by O(1)
+    assert rel_err(g.download(), O.grid(data, iu, iv, co, C, gSize, Sz)) <= TOL
+    gp = ctx.zeros((gSize, gSize), np.complex128)
+    tcv.grid_plain_f64(ctx, vis, None, d_C, gp)
+    ones = np.ones(n, np.complex64)
+    assert rel_err(gp.download(), O.grid_f64(ones, iu, iv, co, C, gSize, Sz)) <= 1e-12
+    rng = np.random.default_rng(Sz)
+    G = (rng.standard_normal((gSize, gSize)) + 1j * rng.standard_normal((gSize, gSize))).astype(np.complex64)
+    out = ctx.empty(n, np.complex128)
+    tcv.degrid_plain_f64(ctx, vis, ctx.to_device(G), d_C, out)
+    assert rel_err(out.download(), O.degrid_f64(G, iu, iv, co, C, Sz)) <= 1e-12
+
+
 def test_out_of_range_visibilities_are_skipped(ctx):
     Sz, gSize = 33, 128
     iu, iv, co, data, C = small_problem(41, 200, gSize, Sz)
@@ -376,19 +402,6 @@ def test_read_bandwidth_probe(ctx):
         ctx.read_bandwidth(ctx.empty(3000, np.uint32))
     with pytest.raises(tcv.TcvError):
         ctx.read_bandwidth(buf, rounds=12)
-
-
-def exact_problem(seed, nVis, gSize, Sz, cluster):
-    """Inputs whose every partial sum is exact in FP32: C taps are multiples
-    of 1/4 in [-1, 1], data and grid values small integers.  Results then do
-    not depend on summation order, so different kernels, plans and the oracle
-    must agree bit for bit."""
-    iu, iv, co, _, C = small_problem(seed, nVis, gSize, Sz, cluster=cluster)
-    rng = np.random.default_rng(seed + 1)
-    C = (rng.integers(-4, 5, C.size) / 4 + 1j * rng.integers(-4, 5, C.size) / 4).astype(np.complex64)
-    data = (rng.integers(-3, 4, nVis) + 1j * rng.integers(-3, 4, nVis)).astype(np.complex64)
-    grid = (rng.integers(-2, 3, (gSize, gSize)) + 1j * rng.integers(-2, 3, (gSize, gSize))).astype(np.complex64)
-    return iu, iv, co, data, C, grid
 
 
 @pytest.mark.parametrize("tw,th", [(64, 64), (32, 64), (16, 32)])

@@ -305,3 +305,30 @@ def test_reference_timing_driver_computes_the_oracle(tmp_path):
         want += abs(O.degrid(g, iu[sl], iv[sl], co[sl], C, S)[0])
     assert r["check"] == pytest.approx(want, rel=1e-5)
     assert r["grid_s"] > 0 and r["degrid_s"] > 0
+
+
+@pytest.mark.parametrize("Sz", [129, 33])
+def test_oracle_refuses_footprints_past_the_grid(Sz):
+    """The oracle's C loops, like the reference's, do no bounds checks, so its
+    wrappers refuse footprints that leave the grid (rows [iv, iv + s) past the
+    last row, columns [iu - s/2, iu - s/2 + s) past either edge) or whose
+    slice leaves C, instead of writing past the output array (a heap overrun
+    that glibc reports later, at an unrelated allocation, as an abort).  The
+    seeded test problems draw rows in [0, gSize - s] for the same reason."""
+    gSize = 384
+    iu, iv, co, data, C = small_problem(11 + Sz, 3000, gSize, Sz, complex_=True)
+    assert iv.min() >= 0 and iv.max() <= gSize - Sz
+    assert (iu - Sz // 2).min() >= 0 and (iu - Sz // 2 + Sz).max() <= gSize
+    O.grid(data, iu, iv, co, C, gSize, Sz)        # in range: accepted
+    for bad in ("row", "col", "slice"):
+        iu2, iv2, co2 = iu.copy(), iv.copy(), co.copy()
+        if bad == "row":
+            iv2[7] = gSize - Sz + 1                  # last footprint row == gSize
+        elif bad == "col":
+            iu2[7] = gSize - Sz // 2                 # last column == gSize
+        else:
+            co2[7] = C.size - Sz * Sz + 1
+        with pytest.raises(ValueError):
+            O.grid(data, iu2, iv2, co2, C, gSize, Sz)
+        with pytest.raises(ValueError):
+            O.degrid(np.zeros((gSize, gSize), np.complex64), iu2, iv2, co2, C, Sz)
