@@ -129,6 +129,14 @@ class Benchmark:
         return self.samples.iu.download(), self.samples.iv.download(), self.samples.cOffset.download()
 
 
+def sort_by_plane(iu, iv, wPlane, cOffset):
+    """setSort(1): the index arrays reordered by w-plane, keeping the input
+    order within each plane (current/tConvolveMPI/Benchmark.cc:275-298, a
+    counting sort; here a stable sort by plane, the same permutation)."""
+    order = np.argsort(np.asarray(wPlane), kind="stable")
+    return tuple(np.ascontiguousarray(np.asarray(a)[order]) for a in (iu, iv, wPlane, cOffset))
+
+
 class CurrentBenchmark:
     """current/tConvolveMPI/Benchmark.h:38-113 on one MI355X."""
 
@@ -185,11 +193,9 @@ class CurrentBenchmark:
         self.numPerPlane = npp
         self.nOutOfRange = bad
         if self.doSort == 1 and self.wSize > 1:
-            # current/tConvolveMPI/Benchmark.cc:275-298: counting sort by w-plane
-            # (like the reference, the data vector is not permuted)
-            h = [a.download() for a in (iu, iv, wp, co)]
-            order = np.argsort(h[2], kind="stable")
-            iu, iv, wp, co = (ctx.to_device(a[order]) for a in h)
+            # like the reference, the data vector is not permuted
+            h = sort_by_plane(*(a.download() for a in (iu, iv, wp, co)))
+            iu, iv, wp, co = (ctx.to_device(a) for a in h)
         self.samples = Visibilities(ctx, iu, iv, co, self.sSize, self.gSize, C.size, wPlane=wp)
         self.plan = None
 

@@ -61,6 +61,8 @@ def _declare(L):
     L.orc_degrid_f64.argtypes = [I, P, I, P, P, P, P, P, I, P, P]
     L.orc_grid_points.restype = ct.c_longlong
     L.orc_grid_points.argtypes = [I, P, P, I]
+    L.orc_sort_by_plane.argtypes = [I, I, P, P, P, P, P]
+    L.orc_grid_window.argtypes = [I, P, P, P, P, P, P, I, P, I, I, I, I, P, I]
     L.orc_cpu_baseline.restype = I
     L.orc_cpu_baseline.argtypes = [I, I, I, P, P, P, P, P, P, I, P, I,
                                    ct.POINTER(D), ct.POINTER(D)]
@@ -192,6 +194,20 @@ def current_initCOffset(u, v, w, wavenumber, uvCellSize, wCellSize, wSize, gSize
     return iu, iv, wp, co, npp
 
 
+def sort_by_plane(iu, iv, wPlane, cOffset, numPerPlane):
+    """setSort(1)'s counting sort by w-plane (current/tConvolveMPI/Benchmark.cc:275-298);
+    returns sorted copies of iu, iv, wPlane, cOffset."""
+    iu, iv, wp, co = (np.array(a, np.int32, copy=True) for a in (iu, iv, wPlane, cOffset))
+    npp = _c(numPerPlane, np.int64)
+    if wp.size and (wp.min() < 0 or wp.max() >= npp.size):
+        raise ValueError("w-plane outside numPerPlane")
+    if int(npp.sum()) != wp.size or not np.array_equal(np.bincount(wp, minlength=npp.size), npp):
+        raise ValueError("numPerPlane does not count the w-planes")
+    if lib().orc_sort_by_plane(wp.size, npp.size, _p(npp), _p(iu), _p(iv), _p(wp), _p(co)):
+        raise MemoryError("orc_sort_by_plane")
+    return iu, iv, wp, co
+
+
 # ---- kernels ----------------------------------------------------------------
 
 def _vis_args(iu, iv, cOffset, wPlane, sSize):
@@ -231,6 +247,26 @@ def grid(data, iu, iv, cOffset, C, gSize, sSize, wPlane=None, out=None):
     _check_footprints(iu, iv, cOffset, wPlane, sSz, sUni, gSize, C.size)
     lib().orc_grid(data.size, _p(data), _p(iu), _p(iv), _p(cOffset), _p(wPlane),
                    _p(sSz), sUni, _p(C), _p(out), gSize)
+    return out
+
+
+def grid_window(data, iu, iv, cOffset, C, gSize, sSize, x0, y0, w, h, wPlane=None, f64=False):
+    """gridKernel's result on the window [x0, x0+w) x [y0, y0+h) only (an h x w
+    complex array, row y - y0): the values grid() gives those cells, bit for
+    bit (grid_f64()'s with f64=True), visiting only the footprint cells inside
+    the window."""
+    data = _c(data, np.complex64)
+    C = _c(C, np.complex64)
+    iu, iv, cOffset, wPlane, sSz = _vis_args(iu, iv, cOffset, wPlane,
+                                             sSize if wPlane is not None else None)
+    sUni = 0 if wPlane is not None else int(sSize)
+    if not (0 <= x0 and 0 <= y0 and w > 0 and h > 0 and x0 + w <= gSize and y0 + h <= gSize):
+        raise ValueError("window outside the grid")
+    _check_footprints(iu, iv, cOffset, wPlane, sSz, sUni, gSize, C.size)
+    out = np.zeros((h, w), np.complex128 if f64 else np.complex64)
+    lib().orc_grid_window(data.size, _p(data), _p(iu), _p(iv), _p(cOffset), _p(wPlane),
+                          _p(sSz), sUni, _p(C), int(x0), int(y0), int(w), int(h), _p(out),
+                          1 if f64 else 0)
     return out
 
 

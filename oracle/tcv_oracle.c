@@ -453,6 +453,86 @@ void orc_degrid_f64(int nVis, const double *grid, int gSize, const int *iu,
     }
 }
 
+/* a7 restricted to a window of the grid: the cells [x0, x0 + w) x [y0, y0 + h)
+ * of gridKernel's result (attic/tConvolveMPI/Benchmark.cc:125-155), into a
+ * window-sized h x w complex array (row y - y0, column x - x0) that it
+ * accumulates into.  Visibilities are taken in index order and every cell
+ * sums them in that order with the same float operations, so each window
+ * value equals orc_grid's value of that cell bit for bit; only the footprint
+ * rows and columns inside the window are visited, which makes a window of a
+ * full-size job cheap.  f64 != 0: double accumulation into a double window
+ * (orc_grid_f64's values). */
+void orc_grid_window(int nVis, const float *data, const int *iu, const int *iv,
+                     const int *cOffset, const int *wPlane, const int *sSize,
+                     int sSizeUniform, const float *C, int x0, int y0, int w, int h,
+                     void *win, int f64)
+{
+    for (int dind = 0; dind < nVis; ++dind) {
+        const int s = vis_size(dind, wPlane, sSize, sSizeUniform);
+        const int u0 = iu[dind] - s / 2, v0 = iv[dind];
+        /* footprint [u0, u0 + s) x [v0, v0 + s) against the window */
+        const int ua = u0 > x0 ? u0 : x0, ub = (u0 + s < x0 + w) ? u0 + s : x0 + w;
+        const int va = v0 > y0 ? v0 : y0, vb = (v0 + s < y0 + h) ? v0 + s : y0 + h;
+        if (ua >= ub || va >= vb) continue;
+        const float dr = data[2 * dind], di = data[2 * dind + 1];
+        for (int y = va; y < vb; y++) {
+            const float *c = &C[2 * ((long)cOffset[dind] + (long)(y - v0) * s + (ua - u0))];
+            const long o = 2 * ((long)(y - y0) * w + (ua - x0));
+            if (f64) {
+                double *g = (double *)win + o;
+                const double ddr = dr, ddi = di;
+                for (int x = 0; x < ub - ua; x++) {
+                    const double cr = c[2 * x], ci = c[2 * x + 1];
+                    g[2 * x] += ddr * cr - ddi * ci;
+                    g[2 * x + 1] += ddr * ci + ddi * cr;
+                }
+            } else {
+                float *g = (float *)win + o;
+                for (int x = 0; x < ub - ua; x++) {
+                    const float cr = c[2 * x], ci = c[2 * x + 1];
+                    const float pr = dr * cr - di * ci;
+                    const float pi = dr * ci + di * cr;
+                    g[2 * x] += pr;
+                    g[2 * x + 1] += pi;
+                }
+            }
+        }
+    }
+}
+
+/* current/tConvolveMPI/Benchmark.cc:275-298: counting sort of the index
+ * arrays by w-plane (setSort(1)): plane k's visibilities go, in their input
+ * order, to the positions starting at the sum of numPerPlane over the planes
+ * before k.  Like the reference, the visibility data is not permuted. */
+int orc_sort_by_plane(int n, int wSize, const long long *numPerPlane, int *iu, int *iv,
+                      int *wPlane, int *cOffset)
+{
+    int *tmp = (int *)malloc(sizeof(int) * 4 * (size_t)(n > 0 ? n : 1));
+    long long *next = (long long *)malloc(sizeof(long long) * (size_t)(wSize > 0 ? wSize : 1));
+    if (!tmp || !next) {
+        free(tmp);
+        free(next);
+        return -1;
+    }
+    memcpy(tmp, iu, sizeof(int) * (size_t)n);
+    memcpy(tmp + n, iv, sizeof(int) * (size_t)n);
+    memcpy(tmp + 2 * (size_t)n, wPlane, sizeof(int) * (size_t)n);
+    memcpy(tmp + 3 * (size_t)n, cOffset, sizeof(int) * (size_t)n);
+    next[0] = 0;
+    for (int k = 1; k < wSize; k++) next[k] = next[k - 1] + numPerPlane[k - 1];
+    for (int i = 0; i < n; i++) {
+        const int k = tmp[2 * (size_t)n + i];
+        const long long j = next[k]++;
+        iu[j] = tmp[i];
+        iv[j] = tmp[(size_t)n + i];
+        wPlane[j] = k;
+        cOffset[j] = tmp[3 * (size_t)n + i];
+    }
+    free(tmp);
+    free(next);
+    return 0;
+}
+
 long long orc_grid_points(int nVis, const int *wPlane, const int *sSize,
                           int sSizeUniform)
 {

@@ -23,6 +23,11 @@
  *   orc_grid / orc_degrid single-visibility patch == C slice, linearity,
  *                         bilinear adjointness (FP64), grid==1 => sum of slice,
  *                         brute force in FP64, compiled-reference checksums
+ *   orc_sort_by_plane     stable (input order kept within a plane), planes
+ *                         contiguous in order, a permutation of the input
+ *   orc_grid_window       bit-identical to orc_grid / orc_grid_f64 on every
+ *                         window cell (windows on edges, corners, straddling),
+ *                         hence to the compiled reference's grid
  *   orc_find_peak         numpy argmax of |re| (first occurrence), ties, zeros
  *   orc_subtract_psf      delta PSF changes one pixel by gain*peak; all-ones
  *                         PSF changes exactly the overlap rectangle; the
@@ -114,6 +119,19 @@ void orc_degrid_f64(int nVis, const double *grid, int gSize, const int *iu,
                     const int *iv, const int *cOffset, const int *wPlane,
                     const int *sSize, int sSizeUniform, const float *C,
                     double *data);
+
+/* a7 on a window [x0, x0 + w) x [y0, y0 + h) of the grid only, into an h x w
+ * complex window (float, or double when f64 != 0) that it accumulates into:
+ * the same values as orc_grid (orc_grid_f64) on those cells, bit for bit. */
+void orc_grid_window(int nVis, const float *data, const int *iu, const int *iv,
+                     const int *cOffset, const int *wPlane, const int *sSize,
+                     int sSizeUniform, const float *C, int x0, int y0, int w, int h,
+                     void *win, int f64);
+
+/* a13: setSort(1) -- current/tConvolveMPI/Benchmark.cc:275-298: counting sort of
+ * iu, iv, wPlane, cOffset by w-plane (stable), in place.  0 on success. */
+int orc_sort_by_plane(int n, int wSize, const long long *numPerPlane, int *iu, int *iv,
+                      int *wPlane, int *cOffset);
 
 /* a10: grid points = sum over visibilities of sSize(w)^2 --
  * current/tConvolveMPI/Benchmark.cc:683-700 */

@@ -12,9 +12,11 @@ from helpers import rel_err, small_problem
 from tcv import setup as S
 
 pytestmark = pytest.mark.gpu
-# FP16 keeps 11 significant bits: each stored value is within 2^-11 of itself
-# (relative), so the accumulated error stays near 1e-3 of the maximum.
-HALF_TOL = 2e-3
+# FP16 keeps 11 significant bits: each stored value is within 2^-12 of itself
+# (relative, normal range).  Measured against FP64 on these seeded problems:
+# 3.7e-4 (grid) and 2.0e-4 (degrid) of the maximum on the small complex job,
+# 1.8e-4 / 3.9e-5 on the config-1 workload; the bar sits at 1.6x the largest.
+HALF_TOL = 6e-4
 
 
 def _half_run(ctx, vis, data, C, nPlanes, planeTaps, gSize, planeStart=None, kernel=0):
@@ -65,32 +67,54 @@ def test_half_variable_width_planes(ctx):
     ps = np.append(b.cOffset0.astype(np.int64), C.size)
     g, out, _ = _half_run(ctx, vis, data, C, b.wSize, 0, b.gSize, planeStart=ps)
     g64 = O.grid_f64(data, iu[sel], iv[sel], co[sel], C, b.gSize, b.sSize, wPlane=wp[sel])
-    assert rel_err(g, g64) <= HALF_TOL
     o64 = O.degrid_f64(g.astype(np.complex128), iu[sel], iv[sel], co[sel], C, b.sSize, wPlane=wp[sel])
-    assert rel_err(out, o64) <= HALF_TOL
+    e, eo = rel_err(g, g64), rel_err(out, o64)
+    print(f"fp16 variable widths vs fp64: grid {e:.2e}, degrid {eo:.2e}")
+    assert e <= HALF_TOL and eo <= HALF_TOL
 
 
-def test_half_attic_errors_reported(ctx):
-    """The attic workload (160 000 samples, 4096^2 grid, width 129) in FP16
-    storage: max and RMS error of the FP32 and FP16 paths
-    against the FP64 oracle."""
-    prob = S.attic_problem()
+@pytest.mark.parametrize("variant", ["reference", "wide"])
+def test_half_attic_errors_reported(ctx, variant):
+    """The config-1 workload (160 000 samples, 4096^2 grid, width 129; the
+    reference generator's [-1000, 1000] m and BASELINE's [-2000, 2000] m) in
+    FP16 storage: max and RMS error of the FP32 and FP16 gridding and
+    degridding against the FP64 oracle (degridding of the FP32 grid, the
+    same input for all three)."""
+    prob = S.attic_problem(variant)
     job = tcv.GridJob(ctx, prob)
     iu, iv, _, co = job.offsets_host()
     planeTaps = 129 * 129 * 64
-    g32 = (job.grid("tiled"), job.grid_d.download())[1]
+    job.zero_grid()
+    job.grid("tiled")
+    g32 = job.grid_d.download()
     g64 = O.grid_f64(prob.data, iu, iv, co, prob.C, 4096, 129)
     Ch = tcv.HalfTable(ctx, job.C, 33, planeTaps)
     dh = tcv.HalfVis(ctx, job.data)
     gh = ctx.zeros((4096, 4096), np.complex64)
     job.plan.grid_half(dh, Ch, gh)
     g16 = gh.download()
-    m = np.abs(g64).max()
-    e32 = np.abs(g32 - g64).max() / m
-    e16 = np.abs(g16 - g64).max() / m
-    r16 = np.sqrt(np.mean(np.abs(g16 - g64) ** 2)) / m
-    print(f"attic workload grid vs fp64: fp32 max {e32:.2e}, fp16 max {e16:.2e} rms {r16:.2e}")
-    assert e32 <= 1e-5 and e16 <= HALF_TOL
+
+    def errs(a, ref):
+        m = np.abs(ref).max()
+        return np.abs(a - ref).max() / m, np.sqrt(np.mean(np.abs(a - ref) ** 2)) / m
+
+    e32, r32 = errs(g32, g64)
+    e16, r16 = errs(g16, g64)
+    job.degrid("tiled")                 # FP32 degridding of the FP32 grid
+    o32 = job.out.download()
+    oh = ctx.empty(prob.nVis, np.complex64)
+    job.plan.degrid_half(job.grid_d, Ch, oh)
+    o16 = oh.download()
+    o64 = O.degrid_f64(g32.astype(np.complex128), iu, iv, co, prob.C, 129)
+    d32, q32 = errs(o32, o64)
+    d16, q16 = errs(o16, o64)
+    print(f"config 1 ({variant}) vs FP64: grid fp32 max {e32:.2e} rms {r32:.2e}, "
+          f"fp16 max {e16:.2e} rms {r16:.2e}; degrid fp32 max {d32:.2e} rms {q32:.2e}, "
+          f"fp16 max {d16:.2e} rms {q16:.2e}")
+    assert e32 <= 1e-5 and d32 <= 1e-5
+    assert e16 <= 3e-4 and d16 <= 1e-4
+    # RMS well below the maximum error: rounding, not a systematic offset
+    assert r16 <= e16 / 2 and q16 <= d16 / 2
 
 
 def test_half_plan_defaults_for_four_byte_taps(ctx):

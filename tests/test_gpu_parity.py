@@ -310,10 +310,11 @@ def test_attic_benchmark_vs_reference(ctx, method):
     assert rel_err(out, O.degrid(gref, iu, iv, co, C, 129)) <= TOL
 
 
-@pytest.mark.parametrize("rt", [1, 3, 2])
+@pytest.mark.parametrize("rt", [0, 1, 3, 2])
 def test_current_benchmark_vs_oracle(ctx, rt):
-    """Variable per-plane widths and complex Fresnel C (runType 1), small
-    kernels with overSample 128 (runType 3), nearest neighbour (runType 2).
+    """Variable per-plane widths and complex Fresnel C (runType 0, the continuum
+    case: 483 planes on a 6440^2 grid; runType 1, spectral), small kernels with
+    overSample 128 (runType 3), nearest neighbour (runType 2).
 
     These ASKAP-layout workloads pile millions of visibilities onto the central
     pixels, where the reference's own sequential FP32 sum is off the exact sum
@@ -347,6 +348,95 @@ def test_current_benchmark_vs_oracle(ctx, rt):
     oref = O.degrid(gref, iu, iv, co, C, b.sSize, wPlane=wp)
     assert rel_err(out, oref) <= TOL
     assert rel_err(out[:256], arrs["outdata_head"]) <= TOL
+
+
+def test_current_benchmark_rt4_large_kernels(ctx):
+    """runType 4: 87 x 87 kernels, 5.4 M visibilities on a 4129^2 grid (4.1e10
+    grid points, too many for a full oracle run in a test).  Every cell of the
+    tiled grid against the device's FP64 reference kernels, windows of the
+    densest centre, a tile-straddling spot and the populated rim against the
+    oracle (FP32 and FP64), and sampled degridded values against the oracle."""
+    b = tcv.CurrentBenchmark(ctx)
+    b.setRunType(4)
+    b.init()
+    meta, _ = golden("current_rt4")
+    assert b.gSize == meta["gSize"] and int(b.sSize[0]) == 87
+    assert b.nPixelsGridded() == meta["nPixelsGridded"] and b.nOutOfRange == 0
+    b.runGrid()
+    g = b.grid_host()
+    g64d = ctx.zeros((b.gSize, b.gSize), np.complex128)
+    tcv.grid_plain_f64(ctx, b.samples, b.data, b.C, g64d)
+    g64 = g64d.download()
+    g64d.free()
+    e = rel_err(g, g64)
+    print(f"runType 4 grid: max error vs FP64 {e:.2e}")
+    assert e <= TOL
+    iu, iv = b.samples.iu.download(), b.samples.iv.download()
+    co = b.samples.cOffset.download()
+    C = b.C.download()
+    ones = np.ones(iu.size, np.complex64)
+    c = b.gSize // 2
+    ys, xs = np.nonzero(np.abs(g64) > 0)
+    k = int(np.argmax(xs))             # the populated cell furthest right
+    rim = (min(int(xs[k]) - 100, b.gSize - 128), min(max(int(ys[k]) - 32, 0), b.gSize - 64))
+    for x0, y0 in ((c - 64, c - 32), (c + 301, c - 517), rim):
+        w32 = O.grid_window(ones, iu, iv, co, C, b.gSize, 87, x0, y0, 128, 64)
+        w64 = O.grid_window(ones, iu, iv, co, C, b.gSize, 87, x0, y0, 128, 64, f64=True)
+        assert w64.any(), (x0, y0)
+        assert rel_err(g[y0:y0 + 64, x0:x0 + 128], w64) <= TOL, (x0, y0)
+        assert rel_err(g64[y0:y0 + 64, x0:x0 + 128], w64) <= 1e-12, (x0, y0)
+        del w32
+    b.runDegrid()
+    out = b.outdata_host()
+    pick = np.sort(np.random.default_rng(9).choice(iu.size, 3000, replace=False))
+    want = O.degrid(g, iu[pick], iv[pick], co[pick], C, 87)
+    assert rel_err(out[pick], want) <= TOL
+
+
+def test_current_benchmark_setsort(ctx):
+    """setSort(1) (current/tConvolveMPI/Benchmark.cc:275-298): the index arrays
+    are the reference's counting sort by w-plane of the unsorted arrays, and
+    the grid (data = 1 everywhere, so order-free) is unchanged within 1e-5."""
+    rt = 1
+    base = tcv.CurrentBenchmark(ctx)
+    base.setRunType(rt)
+    base.init()
+    srt = tcv.CurrentBenchmark(ctx)
+    srt.setRunType(rt)
+    srt.setSort(1)
+    srt.init()
+    h0 = [a.download() for a in (base.samples.iu, base.samples.iv, base.samples.wPlane,
+                                 base.samples.cOffset)]
+    want = O.sort_by_plane(*h0, base.numPerPlane)
+    got = [a.download() for a in (srt.samples.iu, srt.samples.iv, srt.samples.wPlane,
+                                  srt.samples.cOffset)]
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+    assert srt.nPixelsGridded() == base.nPixelsGridded()
+    base.runGrid()
+    srt.runGrid()
+    assert rel_err(srt.grid_host(), base.grid_host()) <= TOL
+
+
+def test_attic_wide_vs_oracle(ctx):
+    """Config 1 as BASELINE states it: u, v, w uniform in [-2000, 2000] m (the
+    reference generator with a 4000 m span), every one of the 33 w-planes in
+    use; the tiled grid and degrid against the oracle's full runs at 1e-5."""
+    prob = S.attic_problem("wide")
+    job = tcv.GridJob(ctx, prob)
+    assert job.nBad == 0
+    assert (job.numPerPlane > 0).all() and job.numPerPlane.size == 33
+    iu, iv, _, co = job.offsets_host()
+    job.zero_grid()
+    job.grid("tiled")
+    g = job.grid_d.download()
+    gref = O.grid(prob.data, iu, iv, co, prob.C, 4096, 129)
+    assert rel_err(g, gref) <= TOL
+    assert rel_err(g, O.grid_f64(prob.data, iu, iv, co, prob.C, 4096, 129)) <= TOL
+    job.grid_d.upload(gref)
+    job.degrid("tiled")
+    out = job.out.download()
+    assert rel_err(out, O.degrid(gref, iu, iv, co, prob.C, 129)) <= TOL
 
 
 def test_cpp_dropin_driver(ctx):

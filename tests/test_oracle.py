@@ -332,3 +332,70 @@ def test_oracle_refuses_footprints_past_the_grid(Sz):
             O.grid(data, iu2, iv2, co2, C, gSize, Sz)
         with pytest.raises(ValueError):
             O.degrid(np.zeros((gSize, gSize), np.complex64), iu2, iv2, co2, C, Sz)
+
+
+def test_grid_window_equals_full_grid_cells():
+    """The windowed oracle gives exactly the full oracle grid's cells (FP32 and
+    FP64), for windows on the grid's corners and edges, inside, and across
+    the footprints of visibilities that only partly overlap them."""
+    Sz, gSize = 65, 300
+    iu, iv, co, data, C = small_problem(61, 1500, gSize, Sz)
+    full = O.grid(data, iu, iv, co, C, gSize, Sz)
+    full64 = O.grid_f64(data, iu, iv, co, C, gSize, Sz)
+    for x0, y0, w, h in ((0, 0, 64, 32), (gSize - 50, gSize - 20, 50, 20), (0, 137, 300, 5),
+                         (101, 0, 7, 300), (140, 151, 128, 64), (299, 299, 1, 1)):
+        win = O.grid_window(data, iu, iv, co, C, gSize, Sz, x0, y0, w, h)
+        assert np.array_equal(win, full[y0:y0 + h, x0:x0 + w]), (x0, y0)
+        win64 = O.grid_window(data, iu, iv, co, C, gSize, Sz, x0, y0, w, h, f64=True)
+        assert np.array_equal(win64, full64[y0:y0 + h, x0:x0 + w]), (x0, y0)
+    # per-plane widths
+    widths = np.array([33, 65, 17], np.int32)
+    rng = np.random.default_rng(62)
+    wp = rng.integers(0, 3, iu.size).astype(np.int32)
+    sizes = widths.astype(np.int64) ** 2 * 64
+    c0 = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    co2 = (c0[wp] + rng.integers(0, 64, iu.size) * widths[wp].astype(np.int64) ** 2).astype(np.int32)
+    C2 = (rng.standard_normal(int(sizes.sum())) + 1j * rng.standard_normal(int(sizes.sum()))).astype(np.complex64)
+    full2 = O.grid(data, iu, iv, co2, C2, gSize, widths, wPlane=wp)
+    win2 = O.grid_window(data, iu, iv, co2, C2, gSize, widths, 90, 200, 100, 60, wPlane=wp)
+    assert np.array_equal(win2, full2[200:260, 90:190])
+    with pytest.raises(ValueError):
+        O.grid_window(data, iu, iv, co, C, gSize, Sz, 250, 0, 64, 64)
+
+
+@pytest.mark.parametrize("rt", [1])
+def test_sort_by_plane_matches_reference_counting_sort(rt):
+    """setSort(1): the product's stable sort by w-plane gives the same arrays as
+    the reference's counting sort (current/tConvolveMPI/Benchmark.cc:275-298)
+    on the full current-generation offsets; the oracle's sort is itself pinned
+    by its defining properties (a permutation, planes contiguous and in order,
+    input order kept inside each plane)."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "askap-benchmarks_amd"))
+    from tcv.benchmark import sort_by_plane
+    p = O.current_params(rt)
+    u, v, w = O.current_uvw(p["baseline"])
+    sS, c0, _ = O.current_initC(p["uvCellSize"], p["wSize"], p["support"], p["overSample"],
+                                p["wCellSize"], table=False)
+    wn = np.array([1.0e9 / 2.998e8])
+    iu, iv, wp, co, npp = O.current_initCOffset(u, v, w, wn, p["uvCellSize"], p["wCellSize"],
+                                                p["wSize"], p["gSize"], p["overSample"], sS, c0)
+    ref = O.sort_by_plane(iu, iv, wp, co, npp)
+    # the oracle's own pins
+    assert np.all(np.diff(ref[2]) >= 0)
+    assert np.array_equal(np.bincount(ref[2], minlength=p["wSize"]), npp)
+    def key(a, b, c, d):   # iu, iv < 2^13, plane < 2^8, cOffset < 2^29
+        return np.sort((d.astype(np.int64) << 34) | (c.astype(np.int64) << 26) |
+                       (b.astype(np.int64) << 13) | a.astype(np.int64))
+    assert max(iu.max(), iv.max()) < 1 << 13 and wp.max() < 1 << 8 and co.max() < 1 << 29
+    assert np.array_equal(key(*ref), key(iu, iv, wp, co))
+    for k in np.unique(wp)[:: max(1, p["wSize"] // 9)]:
+        sel = wp == k
+        start = int(npp[:k].sum())
+        assert np.array_equal(ref[3][start:start + int(npp[k])], co[sel])
+        assert np.array_equal(ref[0][start:start + int(npp[k])], iu[sel])
+    got = sort_by_plane(iu, iv, wp, co)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
