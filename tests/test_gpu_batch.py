@@ -8,7 +8,7 @@ import pytest
 
 import oracle as O
 import tcv
-from helpers import rel_err
+from helpers import exact_problem, rel_err
 from tcv import setup as S
 from tcv.gridder import BatchJobs
 
