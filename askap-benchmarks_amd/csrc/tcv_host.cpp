@@ -101,6 +101,7 @@ tcv_status tcv_create(int device, tcv_ctx **out)
         return fail(TCV_ERR_NOMEM);
     if (hipMemset(ctx->d_counters, 0, 8 * sizeof(unsigned long long)) != hipSuccess)
         return fail(TCV_ERR_DEVICE);
+    noalloc_setup(ctx);
     *out = ctx;
     return TCV_OK;
 }

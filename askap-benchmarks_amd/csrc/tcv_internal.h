@@ -31,7 +31,14 @@ struct tcv_ctx {
     std::map<int, void *> fftPlans;
     void *majorWs = nullptr;
     size_t majorWsBytes = 0;
+    // device address of the fixed scratch array of tcv_grid_noalloc on this
+    // context's device (null, with the reason, when the lookup failed)
+    void *scratch = nullptr;
+    std::string scratchErr;
 };
+
+// Looks up the scratch array's address on the context's device (tcv_noalloc.hip).
+void noalloc_setup(tcv_ctx *ctx);
 
 // Frees the major-cycle resources of a context (tcv_major.hip).
 void major_release(tcv_ctx *ctx);

@@ -87,6 +87,7 @@ SIGS = {
     "tcv_degrid_plain": (I, [P, ct.POINTER(Geometry), P, P, P]),
     "tcv_grid_plain_f64": (I, [P, ct.POINTER(Geometry), P, P, P]),
     "tcv_degrid_plain_f64": (I, [P, ct.POINTER(Geometry), P, P, P]),
+    "tcv_grid_noalloc": (I, [P, ct.POINTER(Geometry), P, P, P]),
     "tcv_range_errors": (I, [P, PLL]),
     "tcv_plan_create": (I, [P, ct.POINTER(Geometry), ct.POINTER(Tuning), ct.POINTER(P)]),
     "tcv_plan_build": (I, [P, P, ct.POINTER(Geometry)]),

@@ -199,6 +199,26 @@ tcv_status tcv_grid_plain_f64(tcv_ctx *ctx, const tcv_geometry *g, const float *
 tcv_status tcv_degrid_plain_f64(tcv_ctx *ctx, const tcv_geometry *g, const float *d_grid,
                                 const float *d_C, double *d_data);
 
+/* Bytes of the fixed device array tcv_grid_noalloc keeps its plan in. */
+#define TCV_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* tcv_grid_plain's arguments and semantics (accumulates into d_grid) computed
+ * by the tiled path, with every buffer the binning plan needs (entries, work
+ * items, key counters, scan partials, queue heads) in a fixed 64 MiB device
+ * array that each GPU holds once for the life of the process, so the call
+ * allocates no device memory.  No counterpart in the reference: it is for
+ * callers that cannot allocate -- code under stream capture, tight serving
+ * loops.  The plan is the default one with its C-slice keys reduced until it
+ * fits; a job whose plan still needs more than TCV_NOALLOC_SCRATCH_BYTES
+ * (roughly 300 000 visibilities of width 129 on a 4096^2 grid) is refused
+ * with TCV_ERR_UNSUPPORTED before anything is queued.  Every call on a device
+ * uses that device's one array: calls on one context's stream run in order,
+ * but calls that overlap on one GPU (two contexts, two streams) race.
+ * Queued; the per-plane widths of a d_wPlane geometry are copied from the
+ * host sSize array by an ordered copy on the stream. */
+tcv_status tcv_grid_noalloc(tcv_ctx *ctx, const tcv_geometry *g, const float *d_data,
+                            const float *d_C, float *d_grid);
+
 /* Waits and returns (then clears) the number of visibilities skipped by the
  * kernels above because their footprint or slice was out of range. */
 tcv_status tcv_range_errors(tcv_ctx *ctx, long long *count);
