@@ -155,7 +155,49 @@ class BatchJobs:
             self.jobs.append(job)
         self.batch = Batch(ctx, [(j["plan"], j["data"], j["C"], j["grid"], j["out"])
                                  for j in self.jobs])
+        self.tuning = tuning
         self.points = int(sum(sp["nVis"] * sp["sSize"] ** 2 for sp in specs))
+
+    def set_tuning(self, tuning: dict | None):
+        """Re-plans every job with `tuning` (one tile shape for the whole batch;
+        missing fields automatic) and rebuilds the batch over the new plans."""
+        from .device import Batch
+        tuning = dict(tuning or {})
+        tuning.setdefault("tile_w", batch_tile_w([sp["sSize"] for sp in self.specs], tuning))
+        self.batch.close()
+        for j in self.jobs:
+            j["plan"].close()
+            j["plan"] = Plan(self.ctx, j["vis"], **tuning)
+        self.batch = Batch(self.ctx, [(j["plan"], j["data"], j["C"], j["grid"], j["out"])
+                                      for j in self.jobs])
+        self.tuning = tuning
+
+    def autotune(self, candidates=None, reps: int = 2):
+        """The batch's autotuner: times the captured batched step (re-bin + grid
+        + degrid of every job) under each candidate tuning and keeps the
+        fastest.  Default candidates: the tile shapes a batch can share (128 x 64
+        with the pair kernels when every kernel is at least 2 wide, 64 x 64 and
+        64 x 32), each job's phases and work items automatic for that shape.
+        Returns (best tuning, [(tuning, ms per step), ...])."""
+        if candidates is None:
+            widths = [sp["sSize"] for sp in self.specs]
+            candidates = [dict(tile_w=64, tile_h=64), dict(tile_w=64, tile_h=32)]
+            if min(widths) >= 2:
+                candidates.insert(0, dict(tile_w=128, tile_h=64, kernel=3, degrid_kernel=3))
+        table = []
+        for cand in candidates:
+            self.set_tuning(cand)
+            self.zero_grids()
+            self.run_batched(graph=True)          # capture + warm
+            self.ctx.sync()
+            self.ctx.timer_start()
+            for _ in range(reps):
+                self.zero_grids()
+                self.run_batched(graph=True)
+            table.append((dict(cand), self.ctx.timer_stop() / reps))
+        best = min(table, key=lambda t: t[1])[0]
+        self.set_tuning(best)
+        return best, table
 
     def zero_grids(self):
         for j in self.jobs:
