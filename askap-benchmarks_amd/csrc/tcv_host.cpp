@@ -473,13 +473,13 @@ tcv_status tcv_range_errors(tcv_ctx *ctx, long long *count)
 // tile.  From 8 phases up a multiple of 8, so the phases split evenly over
 // the XCDs.
 static int default_phases(long long cSize, int tapBytes, int sliceKeys, const tcv_geometry *g, int maxS,
-                          int tw, int th, double bytesPerPhase)
+                          int tw, int th, double bytesPerPhase, double itemEntries)
 {
     double target = std::max(1.0, (double)cSize * tapBytes / bytesPerPhase);
     if (g && tw > 0 && th > 0) {
         const double perVis = (1.0 + (maxS - 1.0) / tw) * (1.0 + (maxS - 1.0) / th);
         const double tiles = (double)((g->gSize + tw - 1) / tw) * ((g->gSize + th - 1) / th);
-        target = std::min(target, std::max(1.0, g->nVis * perVis / tiles / 64.0));
+        target = std::min(target, std::max(1.0, g->nVis * perVis / tiles / itemEntries));
     }
     target = std::min(target, (double)sliceKeys);
     if (target >= 8) {
@@ -492,6 +492,7 @@ static int default_phases(long long cSize, int tapBytes, int sliceKeys, const tc
 }
 
 constexpr double kDegridPhaseBytes = 5.5e6;
+constexpr int kItemEntries = 64;   // entries per work item the phase count keeps at least
 constexpr double kGridPhaseBytes = 5.5e6;   // the gridder may use its own count (grid_phases)
 // 128 x 64 tiles carry half again as many entries per work item: smaller phases
 // (measured optimum 256-320 phases on a 1.1 GB table, against 184-216 for 64 x 64)
@@ -508,6 +509,7 @@ static void auto_tuning(int maxS, const tcv_geometry *g, tcv_tuning *t)
     // faster at width 129, 10-11 % at 113 and 121, 6 % at 97, equal at 81 and
     // 4 % slower at 65.  (FP16 storage keeps 64-wide tiles.)
     if (t->tap_bytes <= 0) t->tap_bytes = 8;
+    if (t->item_entries <= 0) t->item_entries = kItemEntries;
     const bool pairs = minS >= 2 && (t->kernel <= 0 || t->kernel == 3) &&
                        (t->degrid_kernel <= 0 || t->degrid_kernel == 3);
     if (t->tile_w <= 0)
@@ -517,14 +519,15 @@ static void auto_tuning(int maxS, const tcv_geometry *g, tcv_tuning *t)
     if (t->chunk <= 0) t->chunk = 2048;
     const long long slices = std::max<long long>(1, g->cSize / ((long long)minS * minS));
     if (t->slice_keys <= 0) t->slice_keys = (int)std::min<long long>(2048, slices);
-    const bool wide = t->tile_w > 64;
+    // 8192-cell tiles (128 x 64, 64 x 128, 32 x 256) carry more entries per item
+    const bool wide = t->tile_w * t->tile_h > 4096;
     if (t->phases <= 0)
         t->phases = default_phases(g->cSize, t->tap_bytes, t->slice_keys, g, maxS, t->tile_w, t->tile_h,
-                                   wide ? kWidePhaseBytes : kDegridPhaseBytes);
+                                   wide ? kWidePhaseBytes : kDegridPhaseBytes, t->item_entries);
     if (t->xcd_split < 0) t->xcd_split = (t->phases >= 8 && t->phases % 8 == 0) ? 1 : 0;
     if (t->grid_phases <= 0)
         t->grid_phases = default_phases(g->cSize, t->tap_bytes, t->slice_keys, g, maxS, t->tile_w,
-                                        t->tile_h, wide ? kWidePhaseBytes : kGridPhaseBytes);
+                                        t->tile_h, wide ? kWidePhaseBytes : kGridPhaseBytes, t->item_entries);
     if (t->kernel <= 0) t->kernel = kDefaultGridKernel;
     if (t->degrid_kernel <= 0) t->degrid_kernel = kDefaultDegridKernel;
 }
@@ -548,9 +551,10 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
     if (g->gSize > 65535) return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tiled path needs gSize <= 65535");
     if (g->cSize * 8 >= 0x7fffffffLL)
         return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tiled path needs a C table below 2 GiB");
-    tcv_tuning t = tuning ? *tuning : tcv_tuning{0, 0, 0, 0, 0, -1, 0, 0, 0, 0};
+    tcv_tuning t = tuning ? *tuning : tcv_tuning{0, 0, 0, 0, 0, -1, 0, 0, 0, 0, 0};
     if (t.tap_bytes != 0 && t.tap_bytes != 4 && t.tap_bytes != 8)
         return tcv_fail(ctx, TCV_ERR_INVALID, "tuning.tap_bytes must be 0, 4 or 8");
+    if (t.item_entries < 0) return tcv_fail(ctx, TCV_ERR_INVALID, "tuning.item_entries must be >= 0");
     if (!tuning && ctx) {
         auto it = ctx->tuned.find(tuning_class(g));
         if (it != ctx->tuned.end()) t = it->second;
@@ -574,6 +578,7 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
     sh.kern = t.kernel;
     sh.dkern = t.degrid_kernel;
     sh.tapBytes = t.tap_bytes;
+    sh.itemEntries = t.item_entries;
     // the pair kernels need kernel widths of at least 2 (a pair inside each C row)
     int minS = maxS;
     for (int k = 0; k < g->nPlanes; k++) minS = std::min(minS, g->sSize[k]);
@@ -581,9 +586,10 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
         if (sh.kern >= 3) sh.kern = 1;
     }
     if (minS < 2 && sh.dkern >= 3) sh.dkern = 2;
-    if (sh.tw > 64 && (sh.kern != 3 || sh.dkern != 3))
+    if ((sh.tw > 64 || sh.th > 64) && (sh.kern != 3 || sh.dkern != 3))
         return tcv_fail(ctx, TCV_ERR_UNSUPPORTED,
-                        "tiles wider than 64 need the pair kernels (code 3) and kernel widths of at least 2");
+                        "tiles wider or taller than 64 need the pair kernels (code 3) and kernel widths "
+                        "of at least 2");
     sh.minS = minS;
     sh.nTX = (g->gSize + sh.tw - 1) / sh.tw;
     sh.nTY = (g->gSize + sh.th - 1) / sh.th;
@@ -761,7 +767,7 @@ tcv_status tcv_plan_info(tcv_ctx *ctx, tcv_plan *p, tcv_tuning *t, long long *en
 {
     if (!ctx || !p) return TCV_ERR_INVALID;
     if (t) *t = tcv_tuning{p->sh.tw, p->sh.th, p->sh.chunk, p->sh.k2, p->sh.phases, p->sh.xcd, p->sh.kern,
-                           p->sh.dkern, p->sh.phasesG, p->sh.tapBytes};
+                           p->sh.dkern, p->sh.phasesG, p->sh.tapBytes, p->sh.itemEntries};
     int meta[4] = {0, 0, 0, 0};   // items, queue head, entries, out of range
     TCV_HIP(ctx, hipMemcpyAsync(meta, p->meta, sizeof(meta), hipMemcpyDeviceToHost, ctx->stream));
     TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1037,7 +1043,7 @@ std::string tuning_class(const tcv_geometry *g)
 // Tile shapes worth trying for the widest kernel of a job.
 static std::vector<std::pair<int, int>> candidate_tiles(int maxS)
 {
-    if (maxS >= 64) return {{128, 64}, {64, 64}, {64, 32}};
+    if (maxS >= 64) return {{128, 64}, {64, 128}, {32, 256}, {64, 64}, {64, 32}};
     if (maxS >= 48) return {{64, 64}, {64, 32}, {32, 64}};
     if (maxS >= 24) return {{32, 64}, {32, 32}, {64, 32}};
     if (maxS >= 12) return {{16, 64}, {16, 32}, {32, 32}};
@@ -1094,14 +1100,15 @@ tcv_status tcv_autotune(tcv_ctx *ctx, const tcv_geometry *g, const float *d_data
     } else {
         // staged search from the defaults: tile shape x kernel family, then the
         // phase counts of the degridder and gridder items, then the item size
-        tcv_tuning d{0, 0, 0, 0, 0, -1, 0, 0, 0, 0};
+        tcv_tuning d{0, 0, 0, 0, 0, -1, 0, 0, 0, 0, 0};
         auto_tuning(maxS, g, &d);
         int minS = maxS;
         for (int k = 0; k < g->nPlanes; k++) minS = std::min(minS, g->sSize[k]);
         for (auto &ts : candidate_tiles(maxS))
             for (auto kk : {std::make_pair(3, 3), std::make_pair(1, 2), std::make_pair(2, 3)}) {
-                // tiles wider than 64 have only the pair kernels
-                if (ts.first > 64 && (kk.first != 3 || kk.second != 3 || minS < 2)) continue;
+                // tiles wider or taller than 64 have only the pair kernels
+                if ((ts.first > 64 || ts.second > 64) && (kk.first != 3 || kk.second != 3 || minS < 2))
+                    continue;
                 tcv_tuning t = d;
                 t.tile_w = ts.first;
                 t.tile_h = ts.second;

@@ -140,6 +140,7 @@ struct PlanShape {
     int phasesG, xcdG;     // the gridder's work-item phases and XCD order
     int minS;              // narrowest kernel width of the job
     int tapBytes;          // stored tap size the tuning was chosen for (8 or 4)
+    int itemEntries;       // entries per work item the automatic phase counts aimed at
     int nTX, nTY, nTiles;
     int maxS;
     long long maxEntries;  // upper bound on visibility x tile pairs

@@ -1765,15 +1765,19 @@ __global__ __launch_bounds__(NWD * 64) void k_degrid_lds2(ItemSource<BATCH> src)
             else if (mode == 2) walk(std::integral_constant<int, 2>{});
             else walk(std::integral_constant<int, 0>{});
             if ((w & 1) && mode == 0) {
-                // the last column, a lane per row
-                const bool ok = lane < h;
+                // the last column, a lane per row (64 rows a pass: tiles up to
+                // 256 rows high)
                 const unsigned col = (unsigned)(w - 1);
-                const float2 cv = Taps<HALF>::load(
-                    rc, ok ? umad24(lane, sB, cB + col * TB) : kOOB);
-                const float2 g = tile[min((rowLo + lane) * TW + colLo + w - 1, TH * TW - 1)];
-                const float2 gz = ok ? g : make_float2(0.f, 0.f);
-                sr = fmaf(gz.x, cv.x, fmaf(-gz.y, cv.y, sr));
-                si = fmaf(gz.x, cv.y, fmaf(gz.y, cv.x, si));
+                for (int r0 = 0; r0 < h; r0 += 64) {
+                    const int r = r0 + lane;
+                    const bool ok = r < h;
+                    const float2 cv = Taps<HALF>::load(
+                        rc, ok ? umad24(r, sB, cB + col * TB) : kOOB);
+                    const float2 g = tile[min((rowLo + r) * TW + colLo + w - 1, TH * TW - 1)];
+                    const float2 gz = ok ? g : make_float2(0.f, 0.f);
+                    sr = fmaf(gz.x, cv.x, fmaf(-gz.y, cv.y, sr));
+                    si = fmaf(gz.x, cv.y, fmaf(gz.y, cv.x, si));
+                }
             }
             sr = wave_sum(sr);
             si = wave_sum(si);
@@ -2212,12 +2216,20 @@ constexpr int kRB = TCV_RB;   // row groups per uniform range test (loads issued
 #define TCV_RB2 8
 #endif
 constexpr int kRB2 = TCV_RB2;
+// tiles taller than 64 rows: row groups per range test of the pair gridder
+// (a wave's groups there hold several tile rows each)
+#ifndef TCV_RB2_TALL
+#define TCV_RB2_TALL 8
+#endif
+constexpr int kRB2Tall = TCV_RB2_TALL;
 
 template <int TW, int TH>
 struct TileCfg {
-    // tiles wider than a wave have only the pair kernels (kernel 3)
-    static constexpr bool kPairOnly = TW > 64;
-    static constexpr int RPI = kPairOnly ? 1 : 64 / TW;
+    // tiles wider than a wave or taller than 64 rows have only the pair
+    // kernels (kernel 3)
+    static constexpr bool kPairOnly = TW > 64 || TH > 64;
+    static constexpr int RB2 = TH > 64 ? kRB2Tall : kRB2;
+    static constexpr int RPI = TW > 64 ? 1 : 64 / TW;
     static constexpr int RG = kPairOnly ? 0 : TH / (kNW * RPI);   // gridder rows groups per wave
     static constexpr int RD = kPairOnly ? 0 : TH / RPI;           // degridder row groups per wave
     static constexpr int RG2 = TH / (kNW * (128 / TW));   // two-column gridder (0: not built)
@@ -2261,7 +2273,7 @@ tcv_status grid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_kin
                               : k_grid_tiled<TW, T::RG, kNW, (T::RG % kRB == 0 ? kRB : T::RG), HALF, BATCH>;
     if constexpr (T::RG2 >= 1) {
         if (kern_kind >= 3 || T::kPairOnly)
-            kern = k_grid_tiled2<TW, T::RG2, kNW, (T::RG2 % kRB2 == 0 ? kRB2 : T::RG2), HALF, BATCH>;
+            kern = k_grid_tiled2<TW, T::RG2, kNW, (T::RG2 % T::RB2 == 0 ? T::RB2 : T::RG2), HALF, BATCH>;
     }
     if constexpr (HALF && T::RG4 >= 1 && !T::kPairOnly) {
         // FP16 storage: four taps per 16-byte load
@@ -2313,7 +2325,8 @@ tcv_status degrid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_k
 }  // namespace
 
 #define TCV_TILE_CONFIGS(X) \
-    X(128, 64) X(64, 64) X(64, 32) X(32, 64) X(32, 32) X(16, 64) X(16, 32) X(8, 64) X(8, 32)
+    X(128, 64) X(64, 128) X(32, 256) X(64, 64) X(64, 32) X(32, 64) X(32, 32) X(16, 64) X(16, 32) \
+    X(8, 64) X(8, 32)
 
 bool plan_two_level(const PlanShape &sh)
 {

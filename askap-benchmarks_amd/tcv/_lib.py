@@ -55,7 +55,7 @@ class Result(ct.Structure):
 class Tuning(ct.Structure):
     _fields_ = [("tile_w", I), ("tile_h", I), ("chunk", I), ("slice_keys", I),
                 ("phases", I), ("xcd_split", I), ("kernel", I), ("degrid_kernel", I),
-                ("grid_phases", I), ("tap_bytes", I)]
+                ("grid_phases", I), ("tap_bytes", I), ("item_entries", I)]
 
 
 # name -> (restype, argtypes)

@@ -176,14 +176,19 @@ class BatchJobs:
         """The batch's autotuner: times the captured batched step (re-bin + grid
         + degrid of every job) under each candidate tuning and keeps the
         fastest.  Default candidates: the tile shapes a batch can share (128 x 64
-        with the pair kernels when every kernel is at least 2 wide, 64 x 64 and
-        64 x 32), each job's phases and work items automatic for that shape.
+        and 64 x 128 with the pair kernels when every kernel is at least 2 wide,
+        else 64 x 64) and, for each, the fewest entries per work item that each
+        job's automatic C-slice phase count keeps (finer phases keep more of a
+        job's slices in the L2s; each work item zeroes and flushes its tile).
         Returns (best tuning, [(tuning, ms per step), ...])."""
         if candidates is None:
             widths = [sp["sSize"] for sp in self.specs]
-            candidates = [dict(tile_w=64, tile_h=64), dict(tile_w=64, tile_h=32)]
             if min(widths) >= 2:
-                candidates.insert(0, dict(tile_w=128, tile_h=64, kernel=3, degrid_kernel=3))
+                pair = dict(kernel=3, degrid_kernel=3)
+                candidates = ([dict(pair, tile_w=128, tile_h=64, item_entries=n) for n in (64, 40, 32, 24)]
+                              + [dict(pair, tile_w=64, tile_h=128, item_entries=32)])
+            else:
+                candidates = [dict(tile_w=64, tile_h=64, item_entries=n) for n in (64, 32)]
         table = []
         for cand in candidates:
             self.set_tuning(cand)

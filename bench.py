@@ -143,7 +143,8 @@ def bench_batch(ctx, args, ceilings):
         bj.set_tuning(best)
     else:
         best, table = bj.autotune(reps=2)
-    log(f"batch autotune: {best} (" + ", ".join(f"{t.get('tile_w')}x{t.get('tile_h')} {ms:.1f} ms"
+    log(f"batch autotune: {best} (" + ", ".join(f"{t.get('tile_w')}x{t.get('tile_h')}/"
+                                              f"{t.get('item_entries', 0)} {ms:.1f} ms"
                                               for t, ms in table) + ")")
     pts = 2 * bj.points
 
@@ -216,7 +217,7 @@ def bench_batch(ctx, args, ceilings):
 
 # ---- config 1 (attic constants) and FP16 storage (config 4) ------------------------
 
-def bench_config1(ctx, variant, reps):
+def bench_config1(ctx, variant, reps, tuning):
     """Config 1: 160 000 samples, 1 channel, 4096^2 grid, wSize 33, width 129,
     the reference generator (variant "reference": u, v, w in [-1000, 1000] m,
     the P100 figures' inputs; "wide": [-2000, 2000] m).  Tiled grid and degrid
@@ -225,7 +226,7 @@ def bench_config1(ctx, variant, reps):
     import tcv
     from tcv import setup as S
     prob = S.attic_problem(variant)
-    job = tcv.GridJob(ctx, prob)
+    job = tcv.GridJob(ctx, prob, tuning=tuning)
     assert job.nBad == 0
     pts = job.points
     nv, g = prob.nVis, prob.gSize
@@ -635,6 +636,8 @@ def main():
     ap.add_argument("--batch-tuning", default="",
                     help="JSON tuning for the batch instead of its autotuner")
     ap.add_argument("--config1-reps", type=int, default=5)
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the headline batch (and serving and the CPU baseline, which use it)")
     ap.add_argument("--no-spectral", action="store_true")
     ap.add_argument("--no-config1", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -662,8 +665,14 @@ def main():
     args._ceilings = ceilings
     log("read ceilings GB/s: " + ", ".join(f"{k} {v:.0f}" for k, v in ceilings.items()))
 
-    bj, batch = bench_batch(ctx, args, ceilings)
-    result = {
+    if args.no_batch:                  # profiling and tuning runs of the other sections
+        result = {"metric": "grid points/s (grid+degrid) batched vs plain, 1 GPU; p50/p99 job latency",
+                  "value": None, "note": "headline batch not run (--no-batch)"}
+        args.no_serve = args.no_cpu = True
+        bj = None
+    else:
+        bj, batch = bench_batch(ctx, args, ceilings)
+    result = result if args.no_batch else {
         "metric": "grid points/s (grid+degrid) batched vs plain, 1 GPU; p50/p99 job latency",
         "value": batch["value"], "unit": "grid points/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": batch["ms_per_step"], "higher_is_better": True,
@@ -687,8 +696,11 @@ def main():
             f"{cb['per_process'] / 1e6:.0f} per process, {cb['one_core'] / 1e6:.0f} on one core")
     del bj
     ctx.sync()
+    tuning = dict(tile_w=args.tile_w, tile_h=args.tile_h, chunk=args.chunk,
+                  slice_keys=args.slice_keys, phases=args.phases, xcd_split=args.xcd_split,
+                  kernel=args.kernel, degrid_kernel=args.degrid_kernel, grid_phases=args.grid_phases)
     if not args.no_config1:
-        c1 = {v: bench_config1(ctx, v, args.config1_reps) for v in ("reference", "wide")}
+        c1 = {v: bench_config1(ctx, v, args.config1_reps, tuning) for v in ("reference", "wide")}
         result["config1"] = c1
         r = c1["reference"]["fp32"]
         result["config1_grid_Mpts"] = r["grid_Mpts"]
@@ -696,10 +708,6 @@ def main():
         result["p100_grid_Mpts"] = P100_GRID_MPTS
         result["p100_degrid_Mpts"] = P100_DEGRID_MPTS
     if not args.no_spectral:
-        tuning = dict(tile_w=args.tile_w, tile_h=args.tile_h, chunk=args.chunk,
-                      slice_keys=args.slice_keys, phases=args.phases, xcd_split=args.xcd_split,
-                      kernel=args.kernel, degrid_kernel=args.degrid_kernel,
-                      grid_phases=args.grid_phases)
         result["spectral"] = bench_spectral(ctx, args, tuning, args.spectral_steps)
     if not args.no_major:
         result["major_cycle"] = bench_major(ctx, args)

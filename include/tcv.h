@@ -229,7 +229,9 @@ typedef struct {
     int tile_w;      /* 8, 16, 32, 64 or 128 columns (0 = auto: 128 x 64 tiles for
                         kernels 89 wide and up when the pair kernels apply); 128-wide
                         tiles take only the pair kernels (kernel and degrid_kernel 3) */
-    int tile_h;      /* rows per tile: 32 or 64 (64 for 128-wide tiles; 0 = auto) */
+    int tile_h;      /* rows per tile: 32 or 64 for tiles 8..64 wide, 64 for 128-wide
+                        tiles, and the tall shapes 64 x 128 and 32 x 256 (pair kernels
+                        only); 0 = auto */
     int chunk;       /* max entries per work item (0 = auto) */
     int slice_keys;  /* order entries in a tile by C-slice key, 0 = no ordering */
     int phases;      /* >1: split each tile's entries into C-slice phases (0 = auto) */
@@ -249,6 +251,9 @@ typedef struct {
                         storage (0 = 8), 4 FP16 storage (tcv_grid_tiled_half and
                         tcv_degrid_tiled_half); it sets the default tile shape and phase
                         counts, which follow the table's size in bytes */
+    int item_entries; /* fewest entries per work item (on average) the automatic phase
+                        counts keep: finer phases keep a job's C slices in the L2s, but
+                        every work item zeroes and flushes its tile (0 = 64) */
 } tcv_tuning;
 
 typedef struct tcv_plan tcv_plan;

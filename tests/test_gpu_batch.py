@@ -149,3 +149,22 @@ def test_batch_jobs_sharing_one_grid(ctx, tile_w):
         b.grid()
         ctx.sync()
         assert np.array_equal(grid.download(), ref), rep
+
+
+@pytest.mark.parametrize("tw,th,ie", [(64, 128, 32), (32, 256, 16), (128, 64, 8)])
+def test_batch_tall_tiles_and_fine_phases(ctx, tw, th, ie):
+    """Tall tiles (pair kernels only) and automatic phase counts that keep as
+    few as `ie` entries per work item, through the captured batch: every job
+    against the oracle."""
+    bj = BatchJobs(ctx, small_specs(4, seed=23),
+                   tuning=dict(tile_w=tw, tile_h=th, kernel=3, degrid_kernel=3, item_entries=ie))
+    info = bj.jobs[0]["plan"].info()
+    assert (info["tile_w"], info["tile_h"], info["item_entries"]) == (tw, th, ie)
+    bj.zero_grids()
+    bj.run_batched(graph=True)
+    for j in range(len(bj.jobs)):
+        g = bj.jobs[j]["grid"].download()
+        gref, _ = oracle_job(bj, j)
+        assert rel_err(g, gref) <= TOL, j
+        _, oref = oracle_job(bj, j, grid_in=g)
+        assert rel_err(bj.jobs[j]["out"].download(), oref) <= TOL, j

@@ -158,24 +158,26 @@ def test_tiled_shapes_vs_oracle(ctx, tw, th):
         assert rel_err(out, O.degrid(gref, iu, iv, co, C, Sz)) <= TOL, tuning
 
 
+@pytest.mark.parametrize("tw,th", [(128, 64), (64, 128), (32, 256)])
 @pytest.mark.parametrize("Sz,gSize", [(129, 300), (97, 257), (5, 200)])
-def test_wide_tiles_pair_kernels_vs_oracle(ctx, Sz, gSize):
-    """128 x 64 tiles (pair kernels only; the default for kernels 89 wide and
-    up): grid and degrid within 1e-5 of the oracle over item splits,
-    phase counts and the XCD queues, on grids that are not a multiple of the
-    tile (257, 300) and with footprints narrower than the tile."""
+def test_wide_tiles_pair_kernels_vs_oracle(ctx, Sz, gSize, tw, th):
+    """Tiles wider or taller than 64 (pair kernels only; 128 x 64 is the
+    default for kernels 89 wide and up): grid and degrid within 1e-5 of the
+    oracle over item splits, phase counts and the XCD queues, on grids that are
+    not a multiple of the tile (257, 300) and with footprints narrower than the
+    tile."""
     iu, iv, co, data, C = small_problem(160 + Sz, 1500, gSize, Sz, cluster=60)
     vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
     gref = O.grid(data, iu, iv, co, C, gSize, Sz)
     oref = O.degrid(gref, iu, iv, co, C, Sz)
-    for tuning in (dict(tile_w=128, tile_h=64),
-                   dict(tile_w=128, tile_h=64, slice_keys=1, phases=1),
-                   dict(tile_w=128, tile_h=64, chunk=64, slice_keys=256, phases=4),
-                   dict(tile_w=128, tile_h=64, chunk=96, slice_keys=320, phases=16, xcd_split=1),
-                   dict(tile_w=128, tile_h=64, chunk=96, slice_keys=320, phases=16, grid_phases=4,
+    for tuning in (dict(tile_w=tw, tile_h=th),
+                   dict(tile_w=tw, tile_h=th, slice_keys=1, phases=1),
+                   dict(tile_w=tw, tile_h=th, chunk=64, slice_keys=256, phases=4),
+                   dict(tile_w=tw, tile_h=th, chunk=96, slice_keys=320, phases=16, xcd_split=1),
+                   dict(tile_w=tw, tile_h=th, chunk=96, slice_keys=320, phases=16, grid_phases=4,
                         xcd_split=1)):
         info = tcv.Plan(ctx, vis, **tuning).info()
-        assert (info["tile_w"], info["kernel"], info["degrid_kernel"]) == (128, 3, 3)
+        assert (info["tile_w"], info["tile_h"], info["kernel"], info["degrid_kernel"]) == (tw, th, 3, 3)
         g = gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning)
         assert rel_err(g, gref) <= TOL, tuning
         out = gpu_degrid(ctx, vis, gref, C, "tiled", tuning)
@@ -191,7 +193,8 @@ def test_wide_tiles_default_and_rejections(ctx):
     info = tcv.Plan(ctx, vis).info()
     assert (info["tile_w"], info["tile_h"]) == (128, 64)
     assert tcv.Plan(ctx, vis, kernel=1).info()["tile_w"] == 64
-    for bad in (dict(tile_w=128, tile_h=64, kernel=1), dict(tile_w=128, tile_h=64, degrid_kernel=2)):
+    for bad in (dict(tile_w=128, tile_h=64, kernel=1), dict(tile_w=128, tile_h=64, degrid_kernel=2),
+                dict(tile_w=64, tile_h=128, kernel=2), dict(tile_w=32, tile_h=256, degrid_kernel=1)):
         with pytest.raises(tcv.TcvError):
             tcv.Plan(ctx, vis, **bad)
     iu, iv, co, data, C = small_problem(171, 200, 256, 89)
@@ -532,7 +535,7 @@ def test_pair_kernels_every_width_exact(ctx, Sz):
     vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
     gref = O.grid(data, iu, iv, co, C, gSize, Sz)
     oref = O.degrid(G, iu, iv, co, C, Sz)
-    for tw, th in ((128, 64), (64, 64), (32, 64), (16, 32), (8, 64)):
+    for tw, th in ((128, 64), (64, 128), (32, 256), (64, 64), (32, 64), (16, 32), (8, 64)):
         tuning = dict(tile_w=tw, tile_h=th, kernel=3, degrid_kernel=3, phases=4, slice_keys=16, chunk=32)
         assert np.array_equal(gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning), gref), (tw, th)
         assert np.array_equal(gpu_degrid(ctx, vis, G, C, "tiled", tuning), oref), (tw, th)
@@ -563,7 +566,7 @@ def test_pair_kernels_first_and_last_slice_exact(ctx, Sz, keys):
     vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
     gref = O.grid(data, iu, iv, co, C, gSize, Sz)
     oref = O.degrid(G, iu, iv, co, C, Sz)
-    for tw, th in ((128, 64), (64, 64), (32, 64), (16, 32)):
+    for tw, th in ((128, 64), (64, 128), (32, 256), (64, 64), (32, 64), (16, 32)):
         tuning = dict(tile_w=tw, tile_h=th, kernel=3, degrid_kernel=3, phases=min(2, keys),
                       slice_keys=keys, chunk=48)
         assert np.array_equal(gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning), gref), (tw, th)
