@@ -462,6 +462,83 @@ def test_cpp_dropin_driver(ctx):
         assert line[-1] == "0"
 
 
+def _driver_checksums(out):
+    rows = {}
+    for ln in out.splitlines():
+        if ln.startswith("checksum run"):
+            f = ln.split()
+            rows[int(f[2])] = dict(pixels=int(f[4]), grid=complex(float(f[6]), float(f[7])),
+                                   outdata=complex(float(f[9]), float(f[10])), bad=int(f[12]))
+    return rows
+
+
+def test_cpp_current_driver(ctx):
+    """The current tConvolveMPI driver (C++, over the drop-in current Benchmark.h)
+    on runType 1: the gridded-pixel count equals the compiled reference's; the
+    grid's sum equals the FP64 oracle grid's (data = 1: sum_v sum(slice_v))
+    within 2e-6 of the sum of its magnitudes; the degridded data's sum equals
+    sum_p G[p]^2 (the degridding of the gridded grid, bilinear); with setSort(1)
+    the sums are unchanged.  The reference's two-run loop (runType 0 then 1 on
+    one instance) completes and reports both."""
+    import os
+    import subprocess
+    meta, _ = golden("current_rt1")
+    exe = os.path.join(os.path.dirname(tcv.LIB_PATH), "tconvolve_current_gpu")
+    p = O.current_params(1)
+    u, v, w = O.current_uvw(p["baseline"])
+    sS, c0, C = O.current_initC(p["uvCellSize"], p["wSize"], p["support"], p["overSample"],
+                                p["wCellSize"])
+    iu, iv, wp, co, npp = O.current_initCOffset(u, v, w, np.array([1.0e9 / 2.998e8]), p["uvCellSize"],
+                                                p["wCellSize"], p["wSize"], p["gSize"], p["overSample"],
+                                                sS, c0)
+    g64 = O.grid_f64(np.ones(iu.size, np.complex64), iu, iv, co, C, p["gSize"], sS, wPlane=wp)
+    sums = {}
+    for flags in ([], ["--sort"]):
+        r = subprocess.run([exe, "--runtypes", "1", "--checksum"] + flags, capture_output=True, text=True,
+                           timeout=300)
+        print(r.stdout[-2000:])
+        assert r.returncode == 0, r.stderr
+        assert "Spectral gridding performance" in r.stdout and "Spectral degridding performance" in r.stdout
+        row = _driver_checksums(r.stdout)[1]
+        assert row["pixels"] == meta["nPixelsGridded"] and row["bad"] == 0
+        assert abs(row["grid"] - g64.sum()) <= 2e-6 * np.abs(g64).sum(), (row["grid"], g64.sum())
+        want = (g64 * g64).sum()
+        assert abs(row["outdata"] - want) <= 1e-5 * (np.abs(g64) ** 2).sum(), (row["outdata"], want)
+        sums[tuple(flags)] = row
+    r = subprocess.run([exe, "--checksum"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.count("Done") == 2 and "continuum gridding rate (per process)" in r.stdout
+    rows = _driver_checksums(r.stdout)
+    assert set(rows) == {0, 1} and all(x["bad"] == 0 for x in rows.values())
+    assert rows[0]["pixels"] == golden("current_rt0")[0]["nPixelsGridded"]
+
+
+def test_cuda_gridder_replacement(ctx, tmp_path):
+    """The CudaGridder.cc replacement (gridKernelCuda / degridKernelCuda with the
+    signatures of attic/tConvolveCuda/CudaGridder.h:32-50, declared in the
+    calling program itself) against the oracle: gridding accumulates into the
+    given grid, degridding overwrites the data."""
+    import os
+    import subprocess
+    Sz, gSize = 129, 512
+    iu, iv, co, data, C = small_problem(71, 3000, gSize, Sz)
+    rng = np.random.default_rng(72)
+    grid0 = (rng.standard_normal((gSize, gSize)) + 1j * rng.standard_normal((gSize, gSize))).astype(np.complex64)
+    gridin = (rng.standard_normal((gSize, gSize)) + 1j * rng.standard_normal((gSize, gSize))).astype(np.complex64)
+    for name, a in (("iu.i32", iu), ("iv.i32", iv), ("cOffset.i32", co), ("data.c64", data), ("C.c64", C),
+                    ("grid0.c64", grid0), ("gridin.c64", gridin)):
+        a.tofile(str(tmp_path / name))
+    (tmp_path / "params.txt").write_text(f"{gSize} {Sz // 2}\n")
+    exe = os.path.join(os.path.dirname(tcv.LIB_PATH), "cuda_gridder_check")
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "grid_s" in r.stdout and not r.stderr
+    g = np.fromfile(str(tmp_path / "grid.c64"), np.complex64).reshape(gSize, gSize)
+    d = np.fromfile(str(tmp_path / "degrid.c64"), np.complex64)
+    assert rel_err(g, O.grid(data, iu, iv, co, C, gSize, Sz, out=grid0.copy())) <= TOL
+    assert rel_err(d, O.degrid(gridin, iu, iv, co, C, Sz)) <= TOL
+
+
 def test_profiling_counts_launches(ctx):
     Sz, gSize = 33, 128
     iu, iv, co, data, C = small_problem(51, 100, gSize, Sz)

@@ -146,3 +146,38 @@ def test_batch_tile_w_rule():
     assert batch_tile_w([131], dict(kernel=1)) == 64
     assert batch_tile_w([131], dict(degrid_kernel=2)) == 64
     assert batch_tile_w([131], dict(kernel=3, degrid_kernel=3)) == 128
+
+
+REFROOT = "/root/reference"
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REFROOT, "attic", "tConvolveMT")),
+                    reason="the reference tree is not on this machine")
+def test_reference_attic_driver_builds_against_dropin_header(tmp_path):
+    """The reference's own attic tConvolveMT driver (tConvolveMT.cc and its
+    Stopwatch, unmodified, read where they lie) compiles and links against the
+    drop-in include/tconvolve/attic/Benchmark.h and libtcv.so in place of the
+    reference's Benchmark.h and Benchmark.cc: the class name, default
+    constructor, methods and run-size constants it uses all resolve.  (Built
+    here only; nothing of the reference travels or is kept.)"""
+    import shutil
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ missing")
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = os.path.join(REFROOT, "attic", "tConvolveMT")
+    libdir = os.path.join(repo, "askap-benchmarks_amd", "tcv")
+    if not os.path.exists(os.path.join(libdir, "libtcv.so")):
+        pytest.skip("libtcv.so not built")
+    exe = tmp_path / "tConvolveMT_tcv"
+    # -I-: the directories before it serve quoted includes and the source file's
+    # own directory (which holds the reference's Benchmark.h) is not searched
+    subprocess.run([gxx, "-O1", "-std=c++17", "-I", os.path.join(repo, "include", "tconvolve", "attic"),
+                    "-I", os.path.join(repo, "include"), "-I-", "-I", src,
+                    os.path.join(src, "tConvolveMT.cc"), os.path.join(src, "Stopwatch.cc"),
+                    "-o", str(exe), "-L", libdir, "-ltcv", "-lpthread"], check=True,
+                   capture_output=True)
+    assert exe.exists()
+    # its Benchmark methods are the drop-in's, calling into libtcv
+    nm = subprocess.run(["nm", "-C", str(exe)], check=True, capture_output=True, text=True).stdout
+    assert "tcv_create" in nm and "tcv_grid_tiled" in nm
