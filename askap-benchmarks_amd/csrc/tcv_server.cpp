@@ -272,7 +272,7 @@ tcv_status tcv_server_create(tcv_ctx *ctx, const tcv_server_config *cfg, tcv_ser
     s->ctx = ctx;
     s->cfg = *cfg;
     s->tuning = tcv_tuning{cfg->tile_w, cfg->tile_h, cfg->chunk, cfg->slice_keys, cfg->phases, 0,
-                           kDefaultGridKernel, kDefaultDegridKernel};
+                           kDefaultGridKernel, kDefaultDegridKernel, 0, 0, cfg->item_entries};
     if (s->tuning.tile_w <= 0) s->tuning.tile_w = 64;   // one tile shape for every batch
     if (s->tuning.tile_h <= 0) s->tuning.tile_h = 64;
     TCV_HIP(ctx, hipSetDevice(ctx->device));

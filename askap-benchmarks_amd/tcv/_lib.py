@@ -40,7 +40,7 @@ class Job(ct.Structure):
 class ServerConfig(ct.Structure):
     _fields_ = [("arenaBytes", ct.c_size_t), ("maxInflight", I), ("maxBatch", I),
                 ("tile_w", I), ("tile_h", I), ("chunk", I), ("slice_keys", I), ("phases", I),
-                ("autoRelease", I), ("maxBatchPoints", ct.c_longlong)]
+                ("autoRelease", I), ("maxBatchPoints", ct.c_longlong), ("item_entries", I)]
 
 
 class Request(ct.Structure):

@@ -483,12 +483,12 @@ def bench_serving(ctx, args, bj, capacity):
     t = bj.tuning
     srv = tcv.Server(ctx, arena, max_inflight=args.serve_inflight, auto_release=True,
                      max_batch_points=int(args.serve_batch_points), tile_w=t.get("tile_w", 0),
-                     tile_h=t.get("tile_h", 0))
+                     tile_h=t.get("tile_h", 0), item_entries=t.get("item_entries", 0))
     out = {"workload": "Poisson requests over the 64 uneven jobs, 50% grid into new grids, 50% degrid "
                        "resident grids, HBM arena admission, overlapping batches",
            "arena_bytes": arena, "resident_bytes": resident, "budget_bytes": budget,
            "max_inflight": args.serve_inflight, "max_batch_points": args.serve_batch_points,
-           "tile": [t.get("tile_w"), t.get("tile_h")],
+           "tile": [t.get("tile_w"), t.get("tile_h")], "item_entries": t.get("item_entries", 0),
            "capacity_points_per_s": capacity, "mean_points_per_request": mean_pts}
     out["specified"] = serving_stream(srv, bj, mean_pts, capacity, 200.0, args.serve_requests, 1)
     s = out["specified"]

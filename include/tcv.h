@@ -390,6 +390,7 @@ typedef struct {
     int autoRelease;     /* 1: free each result when its request completes */
     long long maxBatchPoints; /* a batch closes once it holds this many grid points, so
                                  large requests run apart from small ones (0 = no limit) */
+    int item_entries;    /* plan tuning: tcv_tuning.item_entries (0 = auto) */
 } tcv_server_config;
 
 typedef struct {
