@@ -176,9 +176,12 @@ def bench_batch(ctx, args, ceilings):
     kern = {k: {"launches": v[0], "avg_ms": v[1] / v[0] if v[0] else None} for k, v in stats.items()}
     dom = max(("grid_batch", "degrid_batch"), key=lambda k: kern[k]["avg_ms"] or 0)
     cells = int(sum(s["gSize"] ** 2 for s in specs))
+    rec = traffic_record("traffic_batch_uneven64")
+    if rec.get("tuning") != best:        # counters measured under another tuning
+        rec = {}
     rf = roofline(dom, kern[dom]["avg_ms"],
                   alg_bytes(bj.points, nVis, cells, "grid" if dom == "grid_batch" else "degrid"),
-                  ceilings, traffic_record("traffic_batch_uneven64"))
+                  ceilings, rec)
 
     def timed(fn, reps):
         ctx.sync()
