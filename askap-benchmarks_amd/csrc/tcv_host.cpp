@@ -516,6 +516,7 @@ static void auto_tuning(int maxS, const tcv_geometry *g, tcv_tuning *t)
         t->tile_w = (maxS >= kWideTileMinWidth && pairs && t->tile_h <= 64 && t->tap_bytes == 8) ? 128
                     : maxS >= 48 ? 64 : maxS >= 24 ? 32 : maxS >= 12 ? 16 : 8;
     if (t->tile_h <= 0) t->tile_h = 64;
+    if (t->grid_cols <= 0) t->grid_cols = auto_grid_cols(t->tile_w, t->tile_h);
     if (t->chunk <= 0) t->chunk = 2048;
     const long long slices = std::max<long long>(1, g->cSize / ((long long)minS * minS));
     if (t->slice_keys <= 0) t->slice_keys = (int)std::min<long long>(2048, slices);
@@ -551,10 +552,11 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
     if (g->gSize > 65535) return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tiled path needs gSize <= 65535");
     if (g->cSize * 8 >= 0x7fffffffLL)
         return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tiled path needs a C table below 2 GiB");
-    tcv_tuning t = tuning ? *tuning : tcv_tuning{0, 0, 0, 0, 0, -1, 0, 0, 0, 0, 0};
+    tcv_tuning t = tuning ? *tuning : tcv_tuning{0, 0, 0, 0, 0, -1, 0, 0, 0, 0, 0, 0};
     if (t.tap_bytes != 0 && t.tap_bytes != 4 && t.tap_bytes != 8)
         return tcv_fail(ctx, TCV_ERR_INVALID, "tuning.tap_bytes must be 0, 4 or 8");
     if (t.item_entries < 0) return tcv_fail(ctx, TCV_ERR_INVALID, "tuning.item_entries must be >= 0");
+    if (t.grid_cols < 0) return tcv_fail(ctx, TCV_ERR_INVALID, "tuning.grid_cols must be >= 0");
     if (!tuning && ctx) {
         auto it = ctx->tuned.find(tuning_class(g));
         if (it != ctx->tuned.end()) t = it->second;
@@ -564,6 +566,10 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
         return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");
     if (t.chunk < 1 || t.phases < 1 || t.slice_keys < 1)
         return tcv_fail(ctx, TCV_ERR_INVALID, "bad tuning");
+    if (!grid_cols_supported(t.tile_w, t.tile_h, t.grid_cols))
+        return tcv_fail(ctx, TCV_ERR_UNSUPPORTED,
+                        "tuning.grid_cols must be tile_w, or tile_w / 2 or / 4 (at least 16) on tiles 64 wide "
+                        "and up with tile_h >= 512 / grid_cols");
     PlanShape &sh = p->sh;
     sh.tw = t.tile_w;
     sh.th = t.tile_h;
@@ -579,6 +585,7 @@ tcv_status plan_setup(tcv_ctx *ctx, const tcv_geometry *g, const tcv_tuning *tun
     sh.dkern = t.degrid_kernel;
     sh.tapBytes = t.tap_bytes;
     sh.itemEntries = t.item_entries;
+    sh.gridCols = t.grid_cols;
     // the pair kernels need kernel widths of at least 2 (a pair inside each C row)
     int minS = maxS;
     for (int k = 0; k < g->nPlanes; k++) minS = std::min(minS, g->sSize[k]);
@@ -767,7 +774,8 @@ tcv_status tcv_plan_info(tcv_ctx *ctx, tcv_plan *p, tcv_tuning *t, long long *en
 {
     if (!ctx || !p) return TCV_ERR_INVALID;
     if (t) *t = tcv_tuning{p->sh.tw, p->sh.th, p->sh.chunk, p->sh.k2, p->sh.phases, p->sh.xcd, p->sh.kern,
-                           p->sh.dkern, p->sh.phasesG, p->sh.tapBytes, p->sh.itemEntries};
+                           p->sh.dkern, p->sh.phasesG, p->sh.tapBytes, p->sh.itemEntries,
+                           p->sh.gridCols};
     int meta[4] = {0, 0, 0, 0};   // items, queue head, entries, out of range
     TCV_HIP(ctx, hipMemcpyAsync(meta, p->meta, sizeof(meta), hipMemcpyDeviceToHost, ctx->stream));
     TCV_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -892,11 +900,12 @@ tcv_status tcv_batch_create(tcv_ctx *ctx, int nJobs, const tcv_job *jobs, tcv_ba
             b->th = J.plan->sh.th;
             b->kern = J.plan->sh.kern;
             b->dkern = J.plan->sh.dkern;
+            b->gridCols = J.plan->sh.gridCols;
         } else if (J.plan->sh.tw != b->tw || J.plan->sh.th != b->th || J.plan->sh.kern != b->kern ||
-                   J.plan->sh.dkern != b->dkern) {
+                   J.plan->sh.dkern != b->dkern || J.plan->sh.gridCols != b->gridCols) {
             delete b;
             return tcv_fail(ctx, TCV_ERR_INVALID,
-                            "tcv_batch_create: all plans of a batch need one tile shape and kernel");
+                            "tcv_batch_create: all plans of a batch need one tile shape, kernel and grid_cols");
         }
         b->plans.push_back(J.plan);
         b->host.push_back(make_job_desc(J.plan, true, J.d_data, J.d_C, (float2 *)J.d_grid, (float2 *)J.d_out, nullptr));
@@ -1099,8 +1108,9 @@ tcv_status tcv_autotune(tcv_ctx *ctx, const tcv_geometry *g, const float *d_data
             if ((st = consider(candidates[k])) != TCV_OK) return st;
     } else {
         // staged search from the defaults: tile shape x kernel family, then the
-        // phase counts of the degridder and gridder items, then the item size
-        tcv_tuning d{0, 0, 0, 0, 0, -1, 0, 0, 0, 0, 0};
+        // pair gridder's column groups, the phase counts of the degridder and
+        // gridder items, then the item size
+        tcv_tuning d{0, 0, 0, 0, 0, -1, 0, 0, 0, 0, 0, 0};
         auto_tuning(maxS, g, &d);
         int minS = maxS;
         for (int k = 0; k < g->nPlanes; k++) minS = std::min(minS, g->sSize[k]);
@@ -1117,10 +1127,19 @@ tcv_status tcv_autotune(tcv_ctx *ctx, const tcv_geometry *g, const float *d_data
                 t.phases = 0;
                 t.grid_phases = 0;
                 t.xcd_split = -1;
-                auto_tuning(maxS, g, &t);   // phases for this tile shape
+                t.grid_cols = 0;
+                auto_tuning(maxS, g, &t);   // phases and column groups for this tile shape
                 if ((st = consider(t)) != TCV_OK) return st;
             }
         if (!any) return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tcv_autotune: no candidate could be planned");
+        const tcv_tuning b0 = *best;
+        if ((op & 1) && b0.kernel == 3)
+            for (int c : {b0.tile_w, b0.tile_w / 2, b0.tile_w / 4}) {
+                if (c == b0.grid_cols || !grid_cols_supported(b0.tile_w, b0.tile_h, c)) continue;
+                tcv_tuning t = b0;
+                t.grid_cols = c;
+                if ((st = consider(t)) != TCV_OK) return st;
+            }
         const tcv_tuning b1 = *best;
         // phase counts around the defaults (multiples of 8 from 8 up)
         auto near = [](int v, double f) {

@@ -133,6 +133,10 @@ constexpr size_t kBatchQueueBytes = 1024;   // ints 0 and 16 + 16 q (q < 8) are 
 constexpr int kDefaultGridKernel = 3;
 constexpr int kDefaultDegridKernel = 3;
 bool grid2_supported(int tw, int th);
+// column-group widths of the pair gridder built for a tile shape, and the
+// automatic choice (64 on 128-wide tiles, else the tile width)
+bool grid_cols_supported(int tw, int th, int cols);
+inline int auto_grid_cols(int tw, int th) { return tw == 128 && grid_cols_supported(tw, th, 64) ? 64 : tw; }
 
 struct PlanShape {
     int tw, th, chunk, k2, phases, xcd;
@@ -141,6 +145,7 @@ struct PlanShape {
     int minS;              // narrowest kernel width of the job
     int tapBytes;          // stored tap size the tuning was chosen for (8 or 4)
     int itemEntries;       // entries per work item the automatic phase counts aimed at
+    int gridCols;          // pair gridder: columns per load instruction (tcv_tuning.grid_cols)
     int nTX, nTY, nTiles;
     int maxS;
     long long maxEntries;  // upper bound on visibility x tile pairs
@@ -190,6 +195,7 @@ bool plan_two_level(const PlanShape &sh);
 struct tcv_batch {
     int nJobs = 0;
     int tw = 0, th = 0, kern = kDefaultGridKernel, dkern = kDefaultDegridKernel;
+    int gridCols = 0;   // pair gridder column groups (PlanShape::gridCols of every plan)
     bool half = false;
     std::vector<JobDesc> host, hostD;   // gridder / degridder descriptors
     std::vector<tcv_plan *> plans;

@@ -1129,20 +1129,28 @@ __global__ __launch_bounds__(NW * 64) void k_degrid_tiled(ItemSource<BATCH> src)
 
 // ---- tiled gridder, two columns per lane ------------------------------------
 // The register-tile gridder with each lane owning two adjacent tile columns,
-// so every C row segment arrives in 16-byte loads (two taps per lane).  Lane
-// l covers columns 2*(l % (TW/2)) and +1 of row l / (TW/2) in each group of
-// 128/TW rows.  A lane whose pair straddles the footprint's left or right
-// edge loads the pair that starts (or ends) on the edge tap and uses one half,
-// so every load stays inside the visibility's C slice.  Same per-cell
-// arithmetic and entry order as the kernels above.
-template <int TW, int R, int NW, int RB, bool HALF, bool BATCH>
+// so every C row segment arrives in 16-byte loads (two taps per lane).  The
+// tile's TW columns are cut into column groups of CGW; one load instruction
+// covers a group's CGW columns over 128 / CGW tile rows (lane l: columns
+// 2*(l % (CGW/2)) and +1 of the group, row l / (CGW/2) of the row group).  A
+// load instruction costs the vector memory path the same whatever the number
+// of its lanes inside the footprint (only one whose lanes are all out of range
+// is cheap), so on footprint pieces narrower than the tile, narrow groups over
+// several rows waste fewer lanes; groups a footprint does not reach are
+// skipped.  A lane whose pair straddles the footprint's left or right edge
+// loads the pair that starts (or ends) on the edge tap and uses one half, so
+// every load stays inside the visibility's C slice.  Same per-cell arithmetic
+// and entry order as the kernels above.
+template <int TW, int CGW, int R, int NW, int RB, bool HALF, bool BATCH>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) void k_grid_tiled2(ItemSource<BATCH> src)
 {
-    constexpr int LPR = TW / 2;      // lanes per tile row
+    constexpr int LPR = CGW / 2;     // lanes per tile row of a column group
     constexpr int RPI = 64 / LPR;    // tile rows per instruction
+    constexpr int NCG = TW / CGW;    // column groups
     constexpr int TH = NW * R * RPI;
     constexpr unsigned TB = Taps<HALF>::kBytes;
     static_assert(R % RB == 0, "row block must divide the register rows");
+    static_assert(NCG * CGW == TW && LPR * RPI == 64, "column groups must tile a tile row");
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lx = 2 * (lane % LPR), ly = lane / LPR;
@@ -1180,9 +1188,11 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
         const bool edgeItem = cload(entries, e0).y < jd.cEdgeLo || cload(entries, e1 - 1).y > jd.cEdgeHi;
         auto run = [&](auto edgeTag) {
         constexpr bool EDGE = decltype(edgeTag)::value;
-        float ar0[R], ai0[R], ar1[R], ai1[R];
+        float ar0[NCG][R], ai0[NCG][R], ar1[NCG][R], ai1[NCG][R];
 #pragma unroll
-        for (int j = 0; j < R; ++j) { ar0[j] = ai0[j] = ar1[j] = ai1[j] = 0.f; }
+        for (int g = 0; g < NCG; ++g)
+#pragma unroll
+            for (int j = 0; j < R; ++j) { ar0[g][j] = ai0[g][j] = ar1[g][j] = ai1[g][j] = 0.f; }
 
         int4 enA = cload(entries, e0);
         int4 enB = cload(entries, min(e0 + 1, e1 - 1));
@@ -1192,67 +1202,78 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
             const int4 enC = cload(entries, min(e + 2, e1 - 1));
             const int u0 = enA.x & 0xFFFF, v0 = (int)((unsigned)enA.x >> 16);
             const int c0 = enA.y, s = entry_width(enA.w);
-            const int cx = Bx + lx - u0;   // C column of the lane's first cell
-            const bool in0 = (unsigned)cx < (unsigned)s, in1 = (unsigned)(cx + 1) < (unsigned)s;
             const int dy = By - v0;
             const int cyl = ly + dy;       // C row of the lane's row in group 0, less r0
             const int sTB = s * (int)TB;
-            if constexpr (!EDGE) {
-                // the pair (cx, cx + 1) even where it straddles the footprint's edge:
-                // the outside tap is the neighbouring one in the table, and its cell
-                // takes it with a zero visibility (an exact zero, tables are finite)
-                const float2 d0 = in0 ? dA : make_float2(0.f, 0.f);
-                const float2 d1 = in1 ? dA : make_float2(0.f, 0.f);
-                const unsigned lanev = (in0 || in1) ? (unsigned)(c0 + cx) * TB : kOOB;
 #pragma unroll
-                for (int jb = 0; jb < R; jb += RB) {
-                    const int r0 = (q0 + jb) * RPI;
-                    if (r0 + RB * RPI + dy > 0 && r0 + dy < s) {
-                        float4 cv[RB];
+            for (int g = 0; g < NCG; ++g) {
+                const int gx = Bx + g * CGW;   // the group's first grid column
+                // a group the footprint's columns [u0, u0 + s) miss would only
+                // load out of range (wave-uniform test)
+                if (NCG > 1 && (gx + CGW <= u0 || gx >= u0 + s)) continue;
+                const int cx = gx + lx - u0;   // C column of the lane's first cell
+                const bool in0 = (unsigned)cx < (unsigned)s, in1 = (unsigned)(cx + 1) < (unsigned)s;
+                if constexpr (!EDGE) {
+                    // the pair (cx, cx + 1) even where it straddles the footprint's edge:
+                    // the outside tap is the neighbouring one in the table, and its cell
+                    // takes it with a zero visibility (an exact zero, tables are finite)
+                    const float2 d0 = in0 ? dA : make_float2(0.f, 0.f);
+                    const float2 d1 = in1 ? dA : make_float2(0.f, 0.f);
+                    const unsigned lanev = (in0 || in1) ? (unsigned)(c0 + cx) * TB : kOOB;
 #pragma unroll
-                        for (int k = 0; k < RB; ++k) {
-                            const int cy = cyl + r0 + k * RPI;
-                            cv[k] = load_pair<HALF>(
-                                rc, (unsigned)cy < (unsigned)s ? umad24(cy, sTB, lanev) : kOOB);
-                        }
+                    for (int jb = 0; jb < R; jb += RB) {
+                        const int r0 = (q0 + jb) * RPI;
+                        if (r0 + RB * RPI + dy > 0 && r0 + dy < s) {
+                            float4 cv[RB];
 #pragma unroll
-                        for (int k = 0; k < RB; ++k) {
-                            ar0[jb + k] = fmaf(d0.x, cv[k].x, fmaf(-d0.y, cv[k].y, ar0[jb + k]));
-                            ai0[jb + k] = fmaf(d0.x, cv[k].y, fmaf(d0.y, cv[k].x, ai0[jb + k]));
-                            ar1[jb + k] = fmaf(d1.x, cv[k].z, fmaf(-d1.y, cv[k].w, ar1[jb + k]));
-                            ai1[jb + k] = fmaf(d1.x, cv[k].w, fmaf(d1.y, cv[k].z, ai1[jb + k]));
+                            for (int k = 0; k < RB; ++k) {
+                                const int cy = cyl + r0 + k * RPI;
+                                cv[k] = load_pair<HALF>(
+                                    rc, (unsigned)cy < (unsigned)s ? umad24(cy, sTB, lanev) : kOOB);
+                            }
+#pragma unroll
+                            for (int k = 0; k < RB; ++k) {
+                                float &r0v = ar0[g][jb + k], &i0v = ai0[g][jb + k];
+                                float &r1v = ar1[g][jb + k], &i1v = ai1[g][jb + k];
+                                r0v = fmaf(d0.x, cv[k].x, fmaf(-d0.y, cv[k].y, r0v));
+                                i0v = fmaf(d0.x, cv[k].y, fmaf(d0.y, cv[k].x, i0v));
+                                r1v = fmaf(d1.x, cv[k].z, fmaf(-d1.y, cv[k].w, r1v));
+                                i1v = fmaf(d1.x, cv[k].w, fmaf(d1.y, cv[k].z, i1v));
+                            }
                         }
                     }
-                }
-            } else {
-                // slice at the table's start or end: load the pair that starts (or
-                // ends) on the edge tap, inside the slice, and pick the halves
-                const int start = min(max(cx, 0), s - 2);
-                const bool sh = cx > start;    // right edge: the wanted tap is the pair's second
-                const unsigned lanev = (in0 || in1) ? (unsigned)(c0 + start) * TB : kOOB;
+                } else {
+                    // slice at the table's start or end: load the pair that starts (or
+                    // ends) on the edge tap, inside the slice, and pick the halves
+                    const int start = min(max(cx, 0), s - 2);
+                    const bool sh = cx > start;    // right edge: the wanted tap is the pair's second
+                    const unsigned lanev = (in0 || in1) ? (unsigned)(c0 + start) * TB : kOOB;
 #pragma unroll
-                for (int jb = 0; jb < R; jb += RB) {
-                    const int r0 = (q0 + jb) * RPI;
-                    if (r0 + RB * RPI + dy > 0 && r0 + dy < s) {
-                        float4 cv[RB];
+                    for (int jb = 0; jb < R; jb += RB) {
+                        const int r0 = (q0 + jb) * RPI;
+                        if (r0 + RB * RPI + dy > 0 && r0 + dy < s) {
+                            float4 cv[RB];
 #pragma unroll
-                        for (int k = 0; k < RB; ++k) {
-                            const int cy = cyl + r0 + k * RPI;
-                            cv[k] = load_pair<HALF>(
-                                rc, (unsigned)cy < (unsigned)s ? umad24(cy, sTB, lanev) : kOOB);
-                        }
+                            for (int k = 0; k < RB; ++k) {
+                                const int cy = cyl + r0 + k * RPI;
+                                cv[k] = load_pair<HALF>(
+                                    rc, (unsigned)cy < (unsigned)s ? umad24(cy, sTB, lanev) : kOOB);
+                            }
 #pragma unroll
-                        for (int k = 0; k < RB; ++k) {
-                            // cell 0 takes tap 0 (tap 1 at the right edge); cell 1 takes
-                            // tap 1 (tap 0 when the pair starts one column late)
-                            float2 t0 = sh ? make_float2(cv[k].z, cv[k].w) : make_float2(cv[k].x, cv[k].y);
-                            float2 t1 = in0 ? make_float2(cv[k].z, cv[k].w) : make_float2(cv[k].x, cv[k].y);
-                            if (!in0) t0 = make_float2(0.f, 0.f);
-                            if (!in1) t1 = make_float2(0.f, 0.f);
-                            ar0[jb + k] = fmaf(dA.x, t0.x, fmaf(-dA.y, t0.y, ar0[jb + k]));
-                            ai0[jb + k] = fmaf(dA.x, t0.y, fmaf(dA.y, t0.x, ai0[jb + k]));
-                            ar1[jb + k] = fmaf(dA.x, t1.x, fmaf(-dA.y, t1.y, ar1[jb + k]));
-                            ai1[jb + k] = fmaf(dA.x, t1.y, fmaf(dA.y, t1.x, ai1[jb + k]));
+                            for (int k = 0; k < RB; ++k) {
+                                // cell 0 takes tap 0 (tap 1 at the right edge); cell 1 takes
+                                // tap 1 (tap 0 when the pair starts one column late)
+                                float2 t0 = sh ? make_float2(cv[k].z, cv[k].w) : make_float2(cv[k].x, cv[k].y);
+                                float2 t1 = in0 ? make_float2(cv[k].z, cv[k].w) : make_float2(cv[k].x, cv[k].y);
+                                if (!in0) t0 = make_float2(0.f, 0.f);
+                                if (!in1) t1 = make_float2(0.f, 0.f);
+                                float &r0v = ar0[g][jb + k], &i0v = ai0[g][jb + k];
+                                float &r1v = ar1[g][jb + k], &i1v = ai1[g][jb + k];
+                                r0v = fmaf(dA.x, t0.x, fmaf(-dA.y, t0.y, r0v));
+                                i0v = fmaf(dA.x, t0.y, fmaf(dA.y, t0.x, i0v));
+                                r1v = fmaf(dA.x, t1.x, fmaf(-dA.y, t1.y, r1v));
+                                i1v = fmaf(dA.x, t1.y, fmaf(dA.y, t1.x, i1v));
+                            }
                         }
                     }
                 }
@@ -1262,34 +1283,37 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
             enB = enC;
         }
 
-        const int col = Bx + lx;
         const int rowsLeft = gSize - (By + q0 * RPI + ly);
-        float2 *__restrict__ g0 = jd.grid + (size_t)(By + q0 * RPI + ly) * gSize + col;
-        const bool c0ok = col < gSize, c1ok = col + 1 < gSize;
         if (it.w && !jd.sharedGrid) {
             // the tile is this item's alone: plain read-modify-write, every read
             // issued before the first write so that their latencies overlap
             // (RB rows at a time: the old values' registers)
 #pragma unroll
-            for (int j0 = 0; j0 < R; j0 += RB) {
-                float2 o0[RB], o1[RB];
+            for (int g = 0; g < NCG; ++g) {
+                const int col = Bx + g * CGW + lx;
+                float2 *__restrict__ g0 = jd.grid + (size_t)(By + q0 * RPI + ly) * gSize + col;
+                const bool c0ok = col < gSize, c1ok = col + 1 < gSize;
 #pragma unroll
-                for (int b = 0; b < RB; ++b) {
-                    const int j = j0 + b;
-                    const float2 *gp = g0 + (size_t)j * RPI * gSize;
-                    const bool rok = j * RPI < rowsLeft;
-                    o0[b] = (rok && c0ok) ? gp[0] : make_float2(0.f, 0.f);
-                    o1[b] = (rok && c1ok) ? gp[1] : make_float2(0.f, 0.f);
-                }
+                for (int j0 = 0; j0 < R; j0 += RB) {
+                    float2 o0[RB], o1[RB];
 #pragma unroll
-                for (int b = 0; b < RB; ++b) {
-                    const int j = j0 + b;
-                    float2 *gp = g0 + (size_t)j * RPI * gSize;
-                    const bool rok = j * RPI < rowsLeft;
-                    if (rok && c0ok && (ar0[j] != 0.f || ai0[j] != 0.f))
-                        gp[0] = make_float2(o0[b].x + ar0[j], o0[b].y + ai0[j]);
-                    if (rok && c1ok && (ar1[j] != 0.f || ai1[j] != 0.f))
-                        gp[1] = make_float2(o1[b].x + ar1[j], o1[b].y + ai1[j]);
+                    for (int b = 0; b < RB; ++b) {
+                        const int j = j0 + b;
+                        const float2 *gp = g0 + (size_t)j * RPI * gSize;
+                        const bool rok = j * RPI < rowsLeft;
+                        o0[b] = (rok && c0ok) ? gp[0] : make_float2(0.f, 0.f);
+                        o1[b] = (rok && c1ok) ? gp[1] : make_float2(0.f, 0.f);
+                    }
+#pragma unroll
+                    for (int b = 0; b < RB; ++b) {
+                        const int j = j0 + b;
+                        float2 *gp = g0 + (size_t)j * RPI * gSize;
+                        const bool rok = j * RPI < rowsLeft;
+                        if (rok && c0ok && (ar0[g][j] != 0.f || ai0[g][j] != 0.f))
+                            gp[0] = make_float2(o0[b].x + ar0[g][j], o0[b].y + ai0[g][j]);
+                        if (rok && c1ok && (ar1[g][j] != 0.f || ai1[g][j] != 0.f))
+                            gp[1] = make_float2(o1[b].x + ar1[g][j], o1[b].y + ai1[g][j]);
+                    }
                 }
             }
         } else {
@@ -1304,8 +1328,10 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
             const int floatsHere = 2 * (gSize - Bx);
 #pragma unroll
             for (int j = 0; j < R; ++j) {
-                *reinterpret_cast<float4 *>(st + ly * 2 * TW + 2 * lx) =
-                    make_float4(ar0[j], ai0[j], ar1[j], ai1[j]);
+#pragma unroll
+                for (int g = 0; g < NCG; ++g)
+                    *reinterpret_cast<float4 *>(st + ly * 2 * TW + 2 * (g * CGW + lx)) =
+                        make_float4(ar0[g][j], ai0[g][j], ar1[g][j], ai1[g][j]);
                 __builtin_amdgcn_wave_barrier();
 #pragma unroll
                 for (int k = 0; k < RPI * 2 * TW / 64; ++k) {
@@ -2263,8 +2289,17 @@ __global__ void k_batch_prefix(const JobDesc *__restrict__ jobs, int nJobs, int 
     queue[16 * t] = 0;   // t = 0: the single queue; t = 1 + q: XCD q's counter
 }
 
+// the pair gridder over column groups of CGW (TH gives every wave whole row groups)
+template <int TW, int TH, int CGW, bool HALF, bool BATCH>
+auto grid2_kernel() -> void (*)(ItemSource<BATCH>)
+{
+    constexpr int R = TH / (kNW * (128 / CGW));
+    constexpr int RB = TileCfg<TW, TH>::RB2;
+    return k_grid_tiled2<TW, CGW, R, kNW, (R % RB == 0 ? RB : R), HALF, BATCH>;
+}
+
 template <int TW, int TH, bool HALF, bool BATCH>
-tcv_status grid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_kind)
+tcv_status grid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_kind, int cols)
 {
     using T = TileCfg<TW, TH>;
     void (*kern)(ItemSource<BATCH>) = nullptr;
@@ -2272,8 +2307,13 @@ tcv_status grid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_kin
         kern = kern_kind == 2 ? k_grid_lds<TW, TH, HALF, BATCH>
                               : k_grid_tiled<TW, T::RG, kNW, (T::RG % kRB == 0 ? kRB : T::RG), HALF, BATCH>;
     if constexpr (T::RG2 >= 1) {
-        if (kern_kind >= 3 || T::kPairOnly)
-            kern = k_grid_tiled2<TW, T::RG2, kNW, (T::RG2 % T::RB2 == 0 ? T::RB2 : T::RG2), HALF, BATCH>;
+        if (kern_kind >= 3 || T::kPairOnly) {
+            kern = grid2_kernel<TW, TH, TW, HALF, BATCH>();
+            if constexpr (TW >= 64 && TH >= kNW * 256 / TW)
+                if (cols == TW / 2) kern = grid2_kernel<TW, TH, TW / 2, HALF, BATCH>();
+            if constexpr (TW >= 64 && TH >= kNW * 512 / TW)
+                if (cols == TW / 4) kern = grid2_kernel<TW, TH, TW / 4, HALF, BATCH>();
+        }
     }
     if constexpr (HALF && T::RG4 >= 1 && !T::kPairOnly) {
         // FP16 storage: four taps per 16-byte load
@@ -2338,6 +2378,13 @@ bool grid2_supported(int tw, int th)
     return th >= kNW * (128 / tw);
 }
 
+bool grid_cols_supported(int tw, int th, int cols)
+{
+    if (cols == tw) return true;
+    if (tw < 64 || cols < 16 || (cols != tw / 2 && cols != tw / 4)) return false;
+    return th >= kNW * 128 / cols && tiled_config_supported(tw, th);
+}
+
 bool tiled_config_supported(int tw, int th)
 {
 #define X(a, b) if (tw == a && th == b) return true;
@@ -2382,8 +2429,8 @@ tcv_status launch_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const void *data, const 
     const int kern = (half && p->sh.kern == 4 && p->sh.minS < 4) ? 3 : p->sh.kern;
 #define X(a, b)                                                                        \
     if (p->sh.tw == a && p->sh.th == b)                                                \
-        return half ? grid_tiled_t<a, b, true, false>(ctx, src, kern)                  \
-                    : grid_tiled_t<a, b, false, false>(ctx, src, kern);
+        return half ? grid_tiled_t<a, b, true, false>(ctx, src, kern, p->sh.gridCols)  \
+                    : grid_tiled_t<a, b, false, false>(ctx, src, kern, p->sh.gridCols);
     TCV_TILE_CONFIGS(X)
 #undef X
     return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");
@@ -2431,7 +2478,7 @@ tcv_status launch_batch(tcv_ctx *ctx, tcv_batch *b, bool degrid)
 #define X(a, c)                                                                        \
     if (b->tw == a && b->th == c)                                                      \
         return degrid ? degrid_tiled_t<a, c, false, true>(ctx, src, b->dkern, true)    \
-                      : grid_tiled_t<a, c, false, true>(ctx, src, b->kern);
+                      : grid_tiled_t<a, c, false, true>(ctx, src, b->kern, b->gridCols);
     TCV_TILE_CONFIGS(X)
 #undef X
     return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");

@@ -66,7 +66,7 @@ extern "C" tcv_status tcv_grid_noalloc(tcv_ctx *ctx, const tcv_geometry *g, cons
     size_t bytes = 0;
     for (int keys = 2048;; keys /= 2) {
         p = tcv_plan{};
-        const tcv_tuning t{0, 0, 0, keys, 0, -1, 0, 0, 0, 0, 0};
+        const tcv_tuning t{0, 0, 0, keys, 0, -1, 0, 0, 0, 0, 0, 0};
         tcv_status st = plan_setup(ctx, g, &t, &p, &bytes);
         if (st != TCV_OK) return st;
         if (bytes <= TCV_NOALLOC_SCRATCH_BYTES || keys == 1) break;

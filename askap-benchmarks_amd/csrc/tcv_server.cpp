@@ -272,9 +272,11 @@ tcv_status tcv_server_create(tcv_ctx *ctx, const tcv_server_config *cfg, tcv_ser
     s->ctx = ctx;
     s->cfg = *cfg;
     s->tuning = tcv_tuning{cfg->tile_w, cfg->tile_h, cfg->chunk, cfg->slice_keys, cfg->phases, 0,
-                           kDefaultGridKernel, kDefaultDegridKernel, 0, 0, cfg->item_entries};
+                           kDefaultGridKernel, kDefaultDegridKernel, 0, 0, cfg->item_entries,
+                           cfg->grid_cols};
     if (s->tuning.tile_w <= 0) s->tuning.tile_w = 64;   // one tile shape for every batch
     if (s->tuning.tile_h <= 0) s->tuning.tile_h = 64;
+    if (s->tuning.grid_cols <= 0) s->tuning.grid_cols = auto_grid_cols(s->tuning.tile_w, s->tuning.tile_h);
     TCV_HIP(ctx, hipSetDevice(ctx->device));
     if (hipMalloc((void **)&s->arenaBase, cfg->arenaBytes) != hipSuccess) {
         (void)hipGetLastError();
@@ -292,6 +294,7 @@ tcv_status tcv_server_create(tcv_ctx *ctx, const tcv_server_config *cfg, tcv_ser
             b->th = s->tuning.tile_h;
             b->kern = s->tuning.kernel;
             b->dkern = s->tuning.degrid_kernel;
+            b->gridCols = s->tuning.grid_cols;
             if (hipMalloc((void **)&b->d_jobs, sizeof(JobDesc) * maxJobs) != hipSuccess ||
                 hipMalloc((void **)&b->d_prefix, sizeof(int) * 9 * (maxJobs + 1)) != hipSuccess ||
                 hipMalloc((void **)&b->d_queue, kBatchQueueBytes) != hipSuccess)

@@ -40,7 +40,8 @@ class Job(ct.Structure):
 class ServerConfig(ct.Structure):
     _fields_ = [("arenaBytes", ct.c_size_t), ("maxInflight", I), ("maxBatch", I),
                 ("tile_w", I), ("tile_h", I), ("chunk", I), ("slice_keys", I), ("phases", I),
-                ("autoRelease", I), ("maxBatchPoints", ct.c_longlong), ("item_entries", I)]
+                ("autoRelease", I), ("maxBatchPoints", ct.c_longlong), ("item_entries", I),
+                ("grid_cols", I)]
 
 
 class Request(ct.Structure):
@@ -55,7 +56,7 @@ class Result(ct.Structure):
 class Tuning(ct.Structure):
     _fields_ = [("tile_w", I), ("tile_h", I), ("chunk", I), ("slice_keys", I),
                 ("phases", I), ("xcd_split", I), ("kernel", I), ("degrid_kernel", I),
-                ("grid_phases", I), ("tap_bytes", I), ("item_entries", I)]
+                ("grid_phases", I), ("tap_bytes", I), ("item_entries", I), ("grid_cols", I)]
 
 
 # name -> (restype, argtypes)

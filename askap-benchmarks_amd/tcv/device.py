@@ -247,7 +247,7 @@ def autotune(ctx: Context, vis: Visibilities, data: DeviceArray, C: DeviceArray,
                                                   c.get("phases", 0), c.get("xcd_split", -1),
                                                   c.get("kernel", 0), c.get("degrid_kernel", 0),
                                                   c.get("grid_phases", 0), c.get("tap_bytes", 0),
-                                                  c.get("item_entries", 0))
+                                                  c.get("item_entries", 0), c.get("grid_cols", 0))
                                            for c in candidates])
         n = len(candidates)
     best = Tuning()
@@ -259,7 +259,7 @@ def autotune(ctx: Context, vis: Visibilities, data: DeviceArray, C: DeviceArray,
                 slice_keys=best.slice_keys, phases=best.phases, xcd_split=best.xcd_split,
                 kernel=best.kernel, degrid_kernel=best.degrid_kernel,
                 grid_phases=best.grid_phases, tap_bytes=best.tap_bytes,
-                item_entries=best.item_entries, ms=ms.value)
+                item_entries=best.item_entries, grid_cols=best.grid_cols, ms=ms.value)
 
 
 class Plan:
@@ -267,14 +267,14 @@ class Plan:
 
     def __init__(self, ctx: Context, vis: Visibilities, tile_w=0, tile_h=0, chunk=0,
                  slice_keys=0, phases=0, xcd_split=-1, kernel=0, degrid_kernel=0, grid_phases=0,
-                 tap_bytes=0, item_entries=0):
+                 tap_bytes=0, item_entries=0, grid_cols=0):
         """tap_bytes=4 makes the automatic choices for FP16 storage (grid_half /
         degrid_half); 0 or 8 for FP32 tables."""
         self.ctx = ctx
         self.vis = vis
         fields = (tile_w, tile_h, chunk, slice_keys, phases, xcd_split, kernel, degrid_kernel,
-                  grid_phases, tap_bytes, item_entries)
-        explicit = fields != (0, 0, 0, 0, 0, -1, 0, 0, 0, 0, 0)
+                  grid_phases, tap_bytes, item_entries, grid_cols)
+        explicit = fields != (0, 0, 0, 0, 0, -1, 0, 0, 0, 0, 0, 0)
         t = Tuning(*fields)
         g = vis.geometry()
         h = P()
@@ -297,7 +297,8 @@ class Plan:
         return dict(tile_w=t.tile_w, tile_h=t.tile_h, chunk=t.chunk, slice_keys=t.slice_keys,
                     phases=t.phases, xcd_split=t.xcd_split, kernel=t.kernel,
                     degrid_kernel=t.degrid_kernel, grid_phases=t.grid_phases,
-                    tap_bytes=t.tap_bytes, item_entries=t.item_entries, entries=e.value,
+                    tap_bytes=t.tap_bytes, item_entries=t.item_entries, grid_cols=t.grid_cols,
+                    entries=e.value,
                     items=it.value,
                     out_of_range=bad.value)
 
@@ -430,11 +431,11 @@ class Server:
 
     def __init__(self, ctx: Context, arena_bytes: int, max_inflight: int = 2, max_batch: int = 0,
                  auto_release: bool = False, tile_w=0, tile_h=0, chunk=0, slice_keys=0, phases=0,
-                 max_batch_points: int = 0, item_entries: int = 0):
+                 max_batch_points: int = 0, item_entries: int = 0, grid_cols: int = 0):
         self.ctx = ctx
         cfg = ServerConfig(int(arena_bytes), max_inflight, max_batch, tile_w, tile_h, chunk,
                            slice_keys, phases, int(auto_release), int(max_batch_points),
-                           int(item_entries))
+                           int(item_entries), int(grid_cols))
         h = P()
         ctx.check(lib().tcv_server_create(ctx.h, ct.byref(cfg), ct.byref(h)), "server_create")
         self.h = h

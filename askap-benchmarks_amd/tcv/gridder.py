@@ -179,13 +179,16 @@ class BatchJobs:
         and 64 x 128 with the pair kernels when every kernel is at least 2 wide,
         else 64 x 64) and, for each, the fewest entries per work item that each
         job's automatic C-slice phase count keeps (finer phases keep more of a
-        job's slices in the L2s; each work item zeroes and flushes its tile).
+        job's slices in the L2s; each work item zeroes and flushes its tile),
+        and on 128 x 64 tiles the pair gridder's columns per load instruction.
         Returns (best tuning, [(tuning, ms per step), ...])."""
         if candidates is None:
             widths = [sp["sSize"] for sp in self.specs]
             if min(widths) >= 2:
                 pair = dict(kernel=3, degrid_kernel=3)
-                candidates = ([dict(pair, tile_w=128, tile_h=64, item_entries=n) for n in (64, 40, 32, 24)]
+                wide = dict(pair, tile_w=128, tile_h=64)
+                candidates = ([dict(wide, item_entries=n, grid_cols=32) for n in (64, 40, 32, 24)]
+                              + [dict(wide, item_entries=32, grid_cols=c) for c in (64, 128)]
                               + [dict(pair, tile_w=64, tile_h=128, item_entries=32)])
             else:
                 candidates = [dict(tile_w=64, tile_h=64, item_entries=n) for n in (64, 32)]

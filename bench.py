@@ -55,7 +55,7 @@ def alg_bytes(points, nVis, gridCells, op):
 
 
 TUNING_KEYS = ("tile_w", "tile_h", "chunk", "slice_keys", "phases", "xcd_split", "kernel",
-               "degrid_kernel", "grid_phases")
+               "degrid_kernel", "grid_phases", "grid_cols")
 
 
 def traffic_record(name):
@@ -144,8 +144,8 @@ def bench_batch(ctx, args, ceilings):
     else:
         best, table = bj.autotune(reps=2)
     log(f"batch autotune: {best} (" + ", ".join(f"{t.get('tile_w')}x{t.get('tile_h')}/"
-                                              f"{t.get('item_entries', 0)} {ms:.1f} ms"
-                                              for t, ms in table) + ")")
+                                              f"{t.get('item_entries', 0)}/c{t.get('grid_cols', 0)} "
+                                              f"{ms:.1f} ms" for t, ms in table) + ")")
     pts = 2 * bj.points
 
     def step():
@@ -486,12 +486,14 @@ def bench_serving(ctx, args, bj, capacity):
     t = bj.tuning
     srv = tcv.Server(ctx, arena, max_inflight=args.serve_inflight, auto_release=True,
                      max_batch_points=int(args.serve_batch_points), tile_w=t.get("tile_w", 0),
-                     tile_h=t.get("tile_h", 0), item_entries=t.get("item_entries", 0))
+                     tile_h=t.get("tile_h", 0), item_entries=t.get("item_entries", 0),
+                     grid_cols=t.get("grid_cols", 0))
     out = {"workload": "Poisson requests over the 64 uneven jobs, 50% grid into new grids, 50% degrid "
                        "resident grids, HBM arena admission, overlapping batches",
            "arena_bytes": arena, "resident_bytes": resident, "budget_bytes": budget,
            "max_inflight": args.serve_inflight, "max_batch_points": args.serve_batch_points,
            "tile": [t.get("tile_w"), t.get("tile_h")], "item_entries": t.get("item_entries", 0),
+           "grid_cols": t.get("grid_cols", 0),
            "capacity_points_per_s": capacity, "mean_points_per_request": mean_pts}
     out["specified"] = serving_stream(srv, bj, mean_pts, capacity, 200.0, args.serve_requests, 1)
     s = out["specified"]

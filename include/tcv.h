@@ -254,6 +254,12 @@ typedef struct {
     int item_entries; /* fewest entries per work item (on average) the automatic phase
                         counts keep: finer phases keep a job's C slices in the L2s, but
                         every work item zeroes and flushes its tile (0 = 64) */
+    int grid_cols;   /* pair gridder (kernel 3, FP32 and FP16 pairs): columns per load
+                        instruction.  One instruction covers grid_cols columns over
+                        128 / grid_cols rows; footprints narrower than the tile waste
+                        fewer lanes with narrow groups.  tile_w, tile_w / 2 or tile_w / 4
+                        (at least 16, and tile_h at least 512 / grid_cols); 0 = auto:
+                        64 on 128-wide tiles, else tile_w */
 } tcv_tuning;
 
 typedef struct tcv_plan tcv_plan;
@@ -391,6 +397,7 @@ typedef struct {
     long long maxBatchPoints; /* a batch closes once it holds this many grid points, so
                                  large requests run apart from small ones (0 = no limit) */
     int item_entries;    /* plan tuning: tcv_tuning.item_entries (0 = auto) */
+    int grid_cols;       /* plan tuning: tcv_tuning.grid_cols (0 = auto) */
 } tcv_server_config;
 
 typedef struct {

@@ -81,3 +81,13 @@ def exact_problem(seed, nVis, gSize, Sz, cluster):
     data = (rng.integers(-3, 4, nVis) + 1j * rng.integers(-3, 4, nVis)).astype(np.complex64)
     grid = (rng.integers(-2, 3, (gSize, gSize)) + 1j * rng.integers(-2, 3, (gSize, gSize))).astype(np.complex64)
     return iu, iv, co, data, C, grid
+
+
+def grid_cols_choices(tw, th):
+    """Column-group widths the pair gridder is built with for a tile shape:
+    the tile width, and half and a quarter of it (at least 16) on tiles 64
+    wide and up whose height gives every wave whole row groups."""
+    out = [tw]
+    if tw >= 64:
+        out += [c for c in (tw // 2, tw // 4) if c >= 16 and th >= 512 // c]
+    return out

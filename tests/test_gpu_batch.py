@@ -151,15 +151,18 @@ def test_batch_jobs_sharing_one_grid(ctx, tile_w):
         assert np.array_equal(grid.download(), ref), rep
 
 
-@pytest.mark.parametrize("tw,th,ie", [(64, 128, 32), (32, 256, 16), (128, 64, 8)])
-def test_batch_tall_tiles_and_fine_phases(ctx, tw, th, ie):
-    """Tall tiles (pair kernels only) and automatic phase counts that keep as
-    few as `ie` entries per work item, through the captured batch: every job
-    against the oracle."""
+@pytest.mark.parametrize("tw,th,ie,cols", [(64, 128, 32, 0), (32, 256, 16, 0), (128, 64, 8, 0),
+                                          (128, 64, 32, 32), (128, 64, 24, 128), (64, 128, 32, 16)])
+def test_batch_tall_tiles_and_fine_phases(ctx, tw, th, ie, cols):
+    """Tall tiles (pair kernels only), automatic phase counts that keep as
+    few as `ie` entries per work item and the pair gridder's column groups,
+    through the captured batch: every job against the oracle."""
     bj = BatchJobs(ctx, small_specs(4, seed=23),
-                   tuning=dict(tile_w=tw, tile_h=th, kernel=3, degrid_kernel=3, item_entries=ie))
+                   tuning=dict(tile_w=tw, tile_h=th, kernel=3, degrid_kernel=3, item_entries=ie,
+                               grid_cols=cols))
     info = bj.jobs[0]["plan"].info()
     assert (info["tile_w"], info["tile_h"], info["item_entries"]) == (tw, th, ie)
+    assert info["grid_cols"] == (cols or (64 if tw == 128 else tw))
     bj.zero_grids()
     bj.run_batched(graph=True)
     for j in range(len(bj.jobs)):

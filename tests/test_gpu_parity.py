@@ -11,7 +11,7 @@ import pytest
 
 import oracle as O
 import tcv
-from helpers import exact_problem, golden, rel_err, small_problem
+from helpers import exact_problem, golden, grid_cols_choices, rel_err, small_problem
 from tcv import setup as S
 
 pytestmark = pytest.mark.gpu
@@ -178,23 +178,32 @@ def test_wide_tiles_pair_kernels_vs_oracle(ctx, Sz, gSize, tw, th):
                         xcd_split=1)):
         info = tcv.Plan(ctx, vis, **tuning).info()
         assert (info["tile_w"], info["tile_h"], info["kernel"], info["degrid_kernel"]) == (tw, th, 3, 3)
-        g = gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning)
-        assert rel_err(g, gref) <= TOL, tuning
         out = gpu_degrid(ctx, vis, gref, C, "tiled", tuning)
         assert rel_err(out, oref) <= TOL, tuning
+        # the gridder on every column-group width (one phase and one key: each
+        # tile's work item is its only one, flushed by plain read-modify-write)
+        for cols in grid_cols_choices(tw, th):
+            g = gpu_grid(ctx, vis, data, C, gSize, "tiled", dict(tuning, grid_cols=cols))
+            assert rel_err(g, gref) <= TOL, (tuning, cols)
 
 
 def test_wide_tiles_default_and_rejections(ctx):
-    """Kernels 89 wide and up get 128 x 64 tiles by default, narrower ones
-    64-wide tiles; 128-wide tiles with a one-column kernel (1, 2) or with
-    width-1 kernels are refused with an error."""
+    """Kernels 89 wide and up get 128 x 64 tiles by default (the pair gridder
+    on 64-column groups), narrower ones 64-wide tiles; 128-wide tiles with a
+    one-column kernel (1, 2) or with width-1 kernels, and column groups that
+    are not built, are refused with an error."""
     iu, iv, co, data, C = small_problem(170, 200, 400, 129)
     vis = vis_from_host(ctx, iu, iv, co, [129], 400, C.size)
     info = tcv.Plan(ctx, vis).info()
-    assert (info["tile_w"], info["tile_h"]) == (128, 64)
+    assert (info["tile_w"], info["tile_h"], info["grid_cols"]) == (128, 64, 64)
     assert tcv.Plan(ctx, vis, kernel=1).info()["tile_w"] == 64
+    assert tcv.Plan(ctx, vis, tile_w=64, tile_h=64).info()["grid_cols"] == 64
+    assert tcv.Plan(ctx, vis, tile_w=128, tile_h=64, grid_cols=32).info()["grid_cols"] == 32
     for bad in (dict(tile_w=128, tile_h=64, kernel=1), dict(tile_w=128, tile_h=64, degrid_kernel=2),
-                dict(tile_w=64, tile_h=128, kernel=2), dict(tile_w=32, tile_h=256, degrid_kernel=1)):
+                dict(tile_w=64, tile_h=128, kernel=2), dict(tile_w=32, tile_h=256, degrid_kernel=1),
+                dict(tile_w=128, tile_h=64, grid_cols=48), dict(tile_w=128, tile_h=64, grid_cols=16),
+                dict(tile_w=32, tile_h=256, grid_cols=16), dict(tile_w=64, tile_h=32, grid_cols=8),
+                dict(tile_w=128, tile_h=64, grid_cols=-1)):
         with pytest.raises(tcv.TcvError):
             tcv.Plan(ctx, vis, **bad)
     iu, iv, co, data, C = small_problem(171, 200, 256, 89)
@@ -612,10 +621,13 @@ def test_pair_kernels_every_width_exact(ctx, Sz):
     vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
     gref = O.grid(data, iu, iv, co, C, gSize, Sz)
     oref = O.degrid(G, iu, iv, co, C, Sz)
-    for tw, th in ((128, 64), (64, 128), (32, 256), (64, 64), (32, 64), (16, 32), (8, 64)):
+    for tw, th in ((128, 64), (64, 128), (32, 256), (64, 64), (64, 32), (32, 64), (16, 32), (8, 64)):
         tuning = dict(tile_w=tw, tile_h=th, kernel=3, degrid_kernel=3, phases=4, slice_keys=16, chunk=32)
-        assert np.array_equal(gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning), gref), (tw, th)
         assert np.array_equal(gpu_degrid(ctx, vis, G, C, "tiled", tuning), oref), (tw, th)
+        # every column-group width of the pair gridder
+        for cols in grid_cols_choices(tw, th):
+            assert np.array_equal(gpu_grid(ctx, vis, data, C, gSize, "tiled", dict(tuning, grid_cols=cols)),
+                                  gref), (tw, th, cols)
 
 
 @pytest.mark.parametrize("Sz,keys", [(3, 4), (9, 4), (33, 4), (129, 4), (9, 4096), (33, 1)])
@@ -646,8 +658,10 @@ def test_pair_kernels_first_and_last_slice_exact(ctx, Sz, keys):
     for tw, th in ((128, 64), (64, 128), (32, 256), (64, 64), (32, 64), (16, 32)):
         tuning = dict(tile_w=tw, tile_h=th, kernel=3, degrid_kernel=3, phases=min(2, keys),
                       slice_keys=keys, chunk=48)
-        assert np.array_equal(gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning), gref), (tw, th)
         assert np.array_equal(gpu_degrid(ctx, vis, G, C, "tiled", tuning), oref), (tw, th)
+        for cols in grid_cols_choices(tw, th):
+            assert np.array_equal(gpu_grid(ctx, vis, data, C, gSize, "tiled", dict(tuning, grid_cols=cols)),
+                                  gref), (tw, th, cols)
 
 
 @pytest.mark.parametrize("tw,th,keys", [(16, 32, 4), (64, 64, 1), (64, 64, 16)])
