@@ -47,6 +47,7 @@ struct Slot {
     hipEvent_t done = nullptr;
     std::vector<long long> reqs;
     bool busy = false;
+    bool alone = false;            // the batch holds a request of exclusivePoints or more
 };
 
 using Clock = std::chrono::steady_clock;
@@ -162,6 +163,7 @@ void scheduler(tcv_server *s)
             const hipError_t q = hipEventQuery(sl.done);
             if (q == hipErrorNotReady) continue;
             std::lock_guard<std::mutex> lk(s->mu);
+            sl.alone = false;
             for (long long id : sl.reqs) {
                 auto it = s->reqs.find(id);
                 if (it != s->reqs.end())
@@ -198,16 +200,38 @@ void scheduler(tcv_server *s)
                 did = true;
             }
         }
-        // launch every admitted request on free slots
+        // launch every admitted request on free slots; a request of
+        // exclusivePoints or more waits for the device to drain and runs alone
+        const long long big = s->cfg.exclusivePoints;
         for (Slot &sl : s->slots) {
             if (sl.busy) continue;
+            bool busyAny = false, aloneAny = false;
+            for (const Slot &o : s->slots) {
+                busyAny = busyAny || o.busy;
+                aloneAny = aloneAny || (o.busy && o.alone);
+            }
+            if (aloneAny) break;
             std::vector<Req *> batch;
             {
                 std::lock_guard<std::mutex> lk(s->mu);
                 long long points = 0;
-                while (!s->admitted.empty() &&
+                if (big > 0 && !s->admitted.empty()) {
+                    Req *h = s->reqs[s->admitted.front()].get();
+                    if (h->points >= big) {
+                        if (busyAny) break;   // drain first
+                        s->admitted.pop_front();
+                        h->state = RUNNING;
+                        h->tLaunch = s->now();
+                        batch.push_back(h);
+                        sl.reqs.push_back(h->id);
+                        sl.alone = true;
+                    }
+                }
+                while (!sl.alone && !s->admitted.empty() &&
                        (s->cfg.maxBatch <= 0 || (int)batch.size() < s->cfg.maxBatch)) {
                     Req *r = s->reqs[s->admitted.front()].get();
+                    // a large request is left for a batch of its own
+                    if (big > 0 && r->points >= big) break;
                     // a batch completes as one: keep large requests apart from small ones
                     if (!batch.empty() && s->cfg.maxBatchPoints > 0 &&
                         points + r->points > s->cfg.maxBatchPoints)
@@ -229,6 +253,7 @@ void scheduler(tcv_server *s)
                 s->err = tcv_last_error(sl.ctx);
                 for (Req *r : batch) finish(s, *r, FAILED, st);
                 sl.reqs.clear();
+                sl.alone = false;
                 s->cv.notify_all();
                 continue;
             }
@@ -265,7 +290,7 @@ extern "C" {
 tcv_status tcv_server_create(tcv_ctx *ctx, const tcv_server_config *cfg, tcv_server **out)
 {
     if (!ctx || !cfg || !out || cfg->arenaBytes < 4096 || cfg->maxInflight < 1 || cfg->maxInflight > 8 ||
-        cfg->maxBatch < 0 || cfg->maxBatchPoints < 0)
+        cfg->maxBatch < 0 || cfg->maxBatchPoints < 0 || cfg->exclusivePoints < 0)
         return tcv_fail(ctx, TCV_ERR_INVALID, "tcv_server_create: bad argument");
     *out = nullptr;
     auto s = std::make_unique<tcv_server>();

@@ -41,7 +41,7 @@ class ServerConfig(ct.Structure):
     _fields_ = [("arenaBytes", ct.c_size_t), ("maxInflight", I), ("maxBatch", I),
                 ("tile_w", I), ("tile_h", I), ("chunk", I), ("slice_keys", I), ("phases", I),
                 ("autoRelease", I), ("maxBatchPoints", ct.c_longlong), ("item_entries", I),
-                ("grid_cols", I)]
+                ("grid_cols", I), ("exclusivePoints", ct.c_longlong)]
 
 
 class Request(ct.Structure):

@@ -431,11 +431,12 @@ class Server:
 
     def __init__(self, ctx: Context, arena_bytes: int, max_inflight: int = 2, max_batch: int = 0,
                  auto_release: bool = False, tile_w=0, tile_h=0, chunk=0, slice_keys=0, phases=0,
-                 max_batch_points: int = 0, item_entries: int = 0, grid_cols: int = 0):
+                 max_batch_points: int = 0, item_entries: int = 0, grid_cols: int = 0,
+                 exclusive_points: int = 0):
         self.ctx = ctx
         cfg = ServerConfig(int(arena_bytes), max_inflight, max_batch, tile_w, tile_h, chunk,
                            slice_keys, phases, int(auto_release), int(max_batch_points),
-                           int(item_entries), int(grid_cols))
+                           int(item_entries), int(grid_cols), int(exclusive_points))
         h = P()
         ctx.check(lib().tcv_server_create(ctx.h, ct.byref(cfg), ct.byref(h)), "server_create")
         self.h = h

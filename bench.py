@@ -490,11 +490,12 @@ def bench_serving(ctx, args, bj, capacity):
     srv = tcv.Server(ctx, arena, max_inflight=args.serve_inflight, auto_release=True,
                      max_batch_points=int(args.serve_batch_points), tile_w=t.get("tile_w", 0),
                      tile_h=t.get("tile_h", 0), item_entries=t.get("item_entries", 0),
-                     grid_cols=t.get("grid_cols", 0))
+                     grid_cols=t.get("grid_cols", 0), exclusive_points=int(args.serve_exclusive_points))
     out = {"workload": "Poisson requests over the 64 uneven jobs, 50% grid into new grids, 50% degrid "
                        "resident grids, HBM arena admission, overlapping batches",
            "arena_bytes": arena, "resident_bytes": resident, "budget_bytes": budget,
            "max_inflight": args.serve_inflight, "max_batch_points": args.serve_batch_points,
+           "exclusive_points": args.serve_exclusive_points,
            "tile": [t.get("tile_w"), t.get("tile_h")], "item_entries": t.get("item_entries", 0),
            "grid_cols": t.get("grid_cols", 0),
            "capacity_points_per_s": capacity, "mean_points_per_request": mean_pts}
@@ -661,6 +662,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-inflight", type=int, default=4)
     ap.add_argument("--serve-batch-points", type=float, default=1e10)
+    ap.add_argument("--serve-exclusive-points", type=float, default=1e10,
+                    help="requests of at least this many grid points run alone (0: never)")
     ap.add_argument("--no-major", action="store_true")
     ap.add_argument("--major-samples", type=int, default=160000)
     ap.add_argument("--major-cycles", type=int, default=5)

@@ -398,6 +398,9 @@ typedef struct {
                                  large requests run apart from small ones (0 = no limit) */
     int item_entries;    /* plan tuning: tcv_tuning.item_entries (0 = auto) */
     int grid_cols;       /* plan tuning: tcv_tuning.grid_cols (0 = auto) */
+    long long exclusivePoints; /* a request of at least this many grid points runs alone: it
+                                  waits for the running batches to finish, and no batch starts
+                                  beside it (0 = never) */
 } tcv_server_config;
 
 typedef struct {

@@ -160,7 +160,44 @@ def test_batch_points_limit(ctx, templates, limit_of):
     srv.close()
 
 
+def test_large_requests_run_alone(ctx, templates):
+    """exclusivePoints: a request of at least that many grid points waits for
+    the running batches to finish and runs with no other batch beside it,
+    while smaller ones still share the device over four slots.  Every result
+    equals its job run alone; the launch and completion times show each large
+    request's run overlapping no other request's."""
+    jobs = templates.jobs
+    pts = sorted(j["spec"]["nVis"] * j["spec"]["sSize"] ** 2 for j in jobs)
+    big = pts[len(pts) // 2]                   # the larger half runs alone
+    srv = tcv.Server(ctx, 8 * GB, max_inflight=4, max_batch_points=2 * pts[-1], exclusive_points=big)
+    order = [(j, op) for _ in range(3) for j in range(len(jobs)) for op in (0, 1)]
+    ids = [(submit(srv, jobs[j], op), j, op) for j, op in order]
+    res = []
+    for rid, j, op in ids:
+        r, out = result_of(srv, rid, jobs[j], op)
+        res.append(r)
+        sp = jobs[j]["spec"]
+        if op == 0:
+            want = ctx.zeros((sp["gSize"], sp["gSize"]), np.complex64)
+            jobs[j]["plan"].grid(jobs[j]["data"], jobs[j]["C"], want)
+        else:
+            want = ctx.empty(sp["nVis"], np.complex64)
+            jobs[j]["plan"].degrid(jobs[j]["grid"], jobs[j]["C"], want)
+        assert rel_err(out, want.download()) <= TOL, (j, op)
+        srv.release(rid)
+    large = [r for r in res if r["points"] >= big]
+    assert large
+    for a in large:
+        for b in res:
+            if b is a:
+                continue
+            assert b["completed"] <= a["launched"] or b["launched"] >= a["completed"], (a, b)
+    srv.close()
+
+
 def test_server_config_rejects_negative_limits(ctx):
+    with pytest.raises(tcv.TcvError):
+        tcv.Server(ctx, GB, exclusive_points=-1)
     with pytest.raises(tcv.TcvError):
         tcv.Server(ctx, GB, max_batch_points=-1)
     with pytest.raises(tcv.TcvError):
