@@ -917,6 +917,68 @@ __device__ __forceinline__ void load_tile(float2 *tile, const float2 *__restrict
     }
 }
 
+// The same tile stored as two planes with a zero column on each side: tile
+// column c is stored column j = c + 1 (j = 0 and j = TW + 1 hold zeros, set by
+// zero_tile_pads), and stored column j sits in plane j & 1 at index
+// row * (TW / 2 + 1) + j / 2.  A pair of adjacent cells is then two 8-byte
+// reads, at one index of both planes or at consecutive ones, each contiguous
+// over the lanes (no bank conflicts, where 16-byte pairs at 8-byte alignment
+// read in two conflicting halves).
+template <int TW, int TH, int NT = 256>
+__device__ __forceinline__ void load_tile_planes(float2 *tile, const float2 *__restrict__ grid, int gSize,
+                                                 int Bx, int By)
+{
+    constexpr int kPairs = TH * TW / 2;
+    constexpr int RS = TW / 2 + 1;        // plane row stride (cells)
+    constexpr int PC = TH * RS;           // cells a plane
+    constexpr int kPer = (kPairs + NT - 1) / NT;
+    constexpr int kBlk = kPer < 8 ? kPer : 8;
+#pragma unroll
+    for (int i0 = 0; i0 < kPer; i0 += kBlk) {
+        float4 v[kBlk];
+#pragma unroll
+        for (int b = 0; b < kBlk; ++b) {
+            const int q = 2 * ((int)threadIdx.x + NT * (i0 + b));
+            const int row = By + q / TW, col = Bx + q % TW;
+            v[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (i0 + b < kPer && q < TH * TW && row < gSize) {
+                const float2 *g = grid + (size_t)row * gSize + col;
+                if (col + 1 < gSize) {
+                    __builtin_memcpy(&v[b], g, 16);
+                } else if (col < gSize) {
+                    const float2 a = *g;
+                    v[b].x = a.x;
+                    v[b].y = a.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < kBlk; ++b) {
+            const int q = 2 * ((int)threadIdx.x + NT * (i0 + b));
+            if (i0 + b < kPer && q < TH * TW) {
+                // cells (r, c) and (r, c + 1), c even: stored columns c + 1 (odd
+                // plane) and c + 2 (even plane)
+                const int i = (q / TW) * RS + (q % TW) / 2;
+                tile[PC + i] = make_float2(v[b].x, v[b].y);
+                tile[i + 1] = make_float2(v[b].z, v[b].w);
+            }
+        }
+        if constexpr (kPer > kBlk) __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// The zero columns of load_tile_planes' layout (stored columns 0 and TW + 1);
+// tile loads never write them.
+template <int TW, int TH>
+__device__ __forceinline__ void zero_tile_pads(float2 *tile)
+{
+    constexpr int RS = TW / 2 + 1, PC = TH * RS;
+    for (int r = threadIdx.x; r < TH; r += blockDim.x) {
+        tile[r * RS] = make_float2(0.f, 0.f);
+        tile[PC + r * RS + TW / 2] = make_float2(0.f, 0.f);
+    }
+}
+
 // ---- tiled gridder ------------------------------------------------------------
 // attic/tConvolveMPI/Benchmark.cc:131-154 reorganised as owner-computes: a
 // workgroup owns a TW x TH grid tile held in VGPRs.  Lane l covers column
@@ -1660,23 +1722,31 @@ __global__ __launch_bounds__(256) void k_degrid_lds(ItemSource<BATCH> src)
 
 // Degridder with the LDS tile and tap pairs: lanes walk each entry's
 // footprint on the tile densely, one pair of adjacent cells per lane and step
-// (a 16-byte C load, two LDS cells).  An odd width takes one more column whose
-// cells count as zero (the taps it reads are neighbours in the table: an
-// exact zero contribution for a finite table); where such a tap could lie
-// outside the table, the last column gets one pass with a lane per row
-// instead.  Index arithmetic is 24-bit (full-rate), and only the last step of
-// an entry checks for lanes past its end.
+// (a 16-byte C load, two LDS cells).  The tile sits in LDS as two planes with
+// a zero column on each side (load_tile_planes), so a pair is two
+// conflict-free 8-byte reads; whether pairs start on an even or an odd stored
+// column is chosen per entry (two compiled variants).  An odd width is walked
+// as pairs over one more column whose cells count as zero: the tile's zero
+// column where the footprint runs past the tile's edge (its tap is a tap of
+// the footprint's own row), else a neighbouring column whose cells are
+// masked (its taps are neighbours in the table: an exact zero contribution
+// for a finite table).  Where such a tap could lie outside the table, the
+// last column gets one pass with a lane per row instead.  Index arithmetic is
+// 24-bit (full-rate), and only the last step of an entry checks for lanes
+// past its end.
 // NWD waves a workgroup: 64 KB tiles (two workgroups a CU) run eight when the
 // job's C-slice phases stay near the L2's size (see degrid_tiled_t)
 template <int TW, int TH, bool HALF, bool BATCH, int NWD = 4>
 __global__ __launch_bounds__(NWD * 64) void k_degrid_lds2(ItemSource<BATCH> src)
 {
     constexpr unsigned TB = Taps<HALF>::kBytes;
-    constexpr unsigned kLastPair = (TH * TW - 2) * 8u;   // byte offset of the tile's last pair
-    __shared__ float2 tile[TH * TW];
+    constexpr int RS = TW / 2 + 1;                    // plane row stride (cells)
+    constexpr unsigned PB = (unsigned)(TH * RS) * 8u; // bytes a plane
+    __shared__ float2 tile[2 * TH * RS];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nItems = src.total();
+    zero_tile_pads<TW, TH>(tile);   // ordered before any read by the barriers below
 
     int next = first_item(src);
     for (;;) {
@@ -1695,7 +1765,7 @@ __global__ __launch_bounds__(NWD * 64) void k_degrid_lds2(ItemSource<BATCH> src)
         const int4 it = cload(jd.items, local);
         const int tyi = it.x / jd.nTX, txi = it.x - tyi * jd.nTX;
         const int Bx = txi * TW, By = tyi * TH;
-        load_tile<TW, TH, NWD * 64>(tile, grid, gSize, Bx, By);
+        load_tile_planes<TW, TH, NWD * 64>(tile, grid, gSize, Bx, By);
         __syncthreads();
 
         const int eEnd = it.y + it.z;
@@ -1707,69 +1777,89 @@ __global__ __launch_bounds__(NWD * 64) void k_degrid_lds2(ItemSource<BATCH> src)
             const int rowLo = max(v0 - By, 0), rowHi = min(v0 + s - By, TH);
             const int w = colHi - colLo, h = rowHi - rowLo;
             if (w <= 0 || h <= 0) continue;
-            // An odd width is walked as pairs over one more column, whose cells take
-            // a zero grid value: the next column when the footprint ends inside the
-            // tile (mode 1; its tap is the next one in the table), else the one
-            // before (mode 2; the tap before).  Where that tap could lie outside the
-            // table, the last column gets a pass of its own (mode 0).
+            // An odd width takes one more column, cells counted as zero:
+            //  mode 3: the tile's zero column at the edge the footprint runs past
+            //          (left: -1, right: TW); its taps lie in the footprint's rows
+            //          unless the footprint ends exactly on that edge, when they
+            //          are the neighbouring taps in the table
+            //  mode 1: the next column (a real one, masked; its tap is the next
+            //          one in the table)
+            //  mode 2: the column before (masked; the tap before)
+            //  mode 0: no added column: the last column gets a pass of its own
+            //          (an added tap could lie outside the table)
             int mode = 0, colA = colLo, wA = w;
             if (w & 1) {
-                if (colHi < TW && c0 + s * s < nTaps) {
+                const bool nextOk = c0 + s * s < nTaps, prevOk = c0 > 0;
+                if (colLo == 0 && colHi < TW && (u0 < Bx || prevOk)) {
+                    mode = 3;
+                    colA = -1;
+                    wA = w + 1;
+                } else if (colHi == TW && colLo > 0 && (u0 + s > Bx + TW || nextOk)) {
+                    mode = 3;
+                    wA = w + 1;
+                } else if (colHi < TW && nextOk) {
                     mode = 1;
                     wA = w + 1;
-                } else if (colLo > 0 && c0 > 0) {
+                } else if (colLo > 0 && prevOk) {
                     mode = 2;
                     colA = colLo - 1;
                     wA = w + 1;
                 }
             }
-            const int maskCol = mode == 1 ? colA + wA - 2 : colA;   // pair's first column
-            const int wp = wA >> 1;                                 // pairs per row
+            const int jA = colA + 1;                 // stored column of the walk's first cell
+            const int wp = wA >> 1;                  // pairs per row
+            const int kEdge = mode == 1 ? wp - 1 : 0;   // the pair with the masked cell
             const int n = wp * h;
             const float rw = 1.0f / (float)max(wp, 1);
-            // byte offsets of cell (rowLo, colA): in C (its tap) and in the tile
+            // byte offsets of cell (rowLo, colA): in C (its tap) and in the even
+            // plane (index rowLo * RS + jA / 2; pair k of a row is k indices on)
             const unsigned cB = (unsigned)(c0 + (By + rowLo - v0) * s + Bx + colA - u0) * TB;
-            const unsigned lB = (unsigned)(rowLo * TW + colA) * 8u;
+            const unsigned lB = (unsigned)(rowLo * RS + (jA >> 1)) * 8u;
             const int sB = s * (int)TB;
             float sr = 0.f, si = 0.f;
             // one step: pair k = p - rl * wp of row rl for p = p0 + lane + 64 u.
             // The offsets of the pair, base + rl * sB + 2 k TB (C) and
-            // base + rl * TW * 8 + 16 k (tile), are written as
+            // base + rl * RS * 8 + 8 k (plane), are written as
             // (base + 2 p TB) + rl * (sB - 2 wp TB): one 24-bit multiply-add each.
             // aL >= 0; aC < 0 when a kernel narrower than the tile gets its added column
-            const int aC = sB - 2 * (int)TB * wp, aL = TW * 8 - 16 * wp;
-            auto step = [&](int p0, auto tailTag, auto modeTag) {
+            const int aC = sB - 2 * (int)TB * wp, aL = RS * 8 - 8 * wp;
+            auto step = [&](int p0, auto tailTag, auto modeTag, auto oddTag) {
                 constexpr bool TAIL = decltype(tailTag)::value;
                 constexpr int MODE = decltype(modeTag)::value;
+                constexpr bool ODD = decltype(oddTag)::value;   // pairs start on odd stored columns
                 float4 c[kLdsU];
                 float2 g0[kLdsU], g1[kLdsU];
                 const int pl = p0 + lane;
                 const float pf = (float)pl + 0.5f;
-                const unsigned cP = cB + (unsigned)pl * (2u * TB), lP = lB + (unsigned)pl * 16u;
+                const unsigned cP = cB + (unsigned)pl * (2u * TB), lP = lB + (unsigned)pl * 8u;
 #pragma unroll
                 for (int u = 0; u < kLdsU; ++u) {
                     // row = floor((p + 0.5) / wp): p + 0.5 is at least 1/(2 wp) from a
                     // multiple of wp, far beyond the rounding of the float arithmetic
                     const int rl = (int)fmaf(pf, rw, (float)(64 * u) * rw);
                     unsigned co = imad24(rl, aC, cP) + (unsigned)(64 * u) * (2u * TB);
-                    unsigned lo = umad24(rl, aL, lP) + (unsigned)(64 * u) * 16u;
+                    unsigned lo = umad24(rl, aL, lP) + (unsigned)(64 * u) * 8u;
                     bool ok = true;
                     if constexpr (TAIL) {
                         ok = pl + 64 * u < n;
                         co = ok ? co : kOOB;
-                        lo = ok ? lo : kLastPair;
+                        lo = ok ? lo : 0u;
                     }
                     c[u] = load_pair<HALF>(rc, co);
-                    const float2 *t = reinterpret_cast<const float2 *>(
-                        reinterpret_cast<const char *>(tile) + lo);
-                    g0[u] = t[0];
-                    g1[u] = t[1];
+                    const char *t = reinterpret_cast<const char *>(tile) + lo;
+                    if constexpr (ODD) {
+                        g0[u] = *reinterpret_cast<const float2 *>(t + PB);
+                        g1[u] = *reinterpret_cast<const float2 *>(t + 8);
+                    } else {
+                        g0[u] = *reinterpret_cast<const float2 *>(t);
+                        g1[u] = *reinterpret_cast<const float2 *>(t + PB);
+                    }
                     if constexpr (TAIL) {
                         if (!ok) g0[u] = g1[u] = make_float2(0.f, 0.f);
                     }
-                    if constexpr (MODE != 0) {
+                    if constexpr (MODE == 1 || MODE == 2) {
                         // the pair on the added column: its outside cell takes zero
-                        const bool edge = (int)((lo >> 3) & (TW - 1)) == maskCol;
+                        const bool edge = imad24(rl, -wp, pl) == kEdge - 64 * u;
                         if (MODE == 1 && edge) g1[u] = make_float2(0.f, 0.f);
                         if (MODE == 2 && edge) g0[u] = make_float2(0.f, 0.f);
                     }
@@ -1782,24 +1872,30 @@ __global__ __launch_bounds__(NWD * 64) void k_degrid_lds2(ItemSource<BATCH> src)
                     si = fmaf(g1[u].x, c[u].w, fmaf(g1[u].y, c[u].z, si));
                 }
             };
-            auto walk = [&](auto modeTag) {
+            auto walk = [&](auto modeTag, auto oddTag) {
                 int p0 = 0;
-                for (; p0 + 64 * kLdsU <= n; p0 += 64 * kLdsU) step(p0, std::false_type{}, modeTag);
-                if (p0 < n) step(p0, std::true_type{}, modeTag);
+                for (; p0 + 64 * kLdsU <= n; p0 += 64 * kLdsU) step(p0, std::false_type{}, modeTag, oddTag);
+                if (p0 < n) step(p0, std::true_type{}, modeTag, oddTag);
             };
-            if (mode == 1) walk(std::integral_constant<int, 1>{});
-            else if (mode == 2) walk(std::integral_constant<int, 2>{});
-            else walk(std::integral_constant<int, 0>{});
+            auto walkMode = [&](auto oddTag) {
+                if (mode == 1) walk(std::integral_constant<int, 1>{}, oddTag);
+                else if (mode == 2) walk(std::integral_constant<int, 2>{}, oddTag);
+                else walk(std::integral_constant<int, 0>{}, oddTag);   // modes 0 and 3: nothing masked
+            };
+            if (jA & 1) walkMode(std::true_type{});
+            else walkMode(std::false_type{});
             if ((w & 1) && mode == 0) {
                 // the last column, a lane per row (64 rows a pass: tiles up to
                 // 256 rows high)
                 const unsigned col = (unsigned)(w - 1);
+                const int j = colLo + w;   // its stored column
+                const float2 *plane = tile + (j & 1) * (TH * RS) + (j >> 1);
                 for (int r0 = 0; r0 < h; r0 += 64) {
                     const int r = r0 + lane;
                     const bool ok = r < h;
                     const float2 cv = Taps<HALF>::load(
                         rc, ok ? umad24(r, sB, cB + col * TB) : kOOB);
-                    const float2 g = tile[min((rowLo + r) * TW + colLo + w - 1, TH * TW - 1)];
+                    const float2 g = plane[min(rowLo + r, TH - 1) * RS];
                     const float2 gz = ok ? g : make_float2(0.f, 0.f);
                     sr = fmaf(gz.x, cv.x, fmaf(-gz.y, cv.y, sr));
                     si = fmaf(gz.x, cv.y, fmaf(gz.y, cv.x, si));
