@@ -645,6 +645,8 @@ def main():
     ap.add_argument("--kernel", type=int, default=0,
                     help="spectral gridder: 1 register tile, 2 LDS tile, 3 register tile with tap pairs")
     ap.add_argument("--grid-phases", type=int, default=0)
+    ap.add_argument("--grid-cols", type=int, default=0,
+                    help="spectral pair gridder: tile columns per load instruction (0: automatic)")
     ap.add_argument("--degrid-kernel", type=int, default=0,
                     help="spectral degridder: 1 register tile, 2 LDS tile, 3 LDS tile with tap pairs")
     ap.add_argument("--batch-jobs", type=int, default=64)
@@ -715,7 +717,8 @@ def main():
     ctx.sync()
     tuning = dict(tile_w=args.tile_w, tile_h=args.tile_h, chunk=args.chunk,
                   slice_keys=args.slice_keys, phases=args.phases, xcd_split=args.xcd_split,
-                  kernel=args.kernel, degrid_kernel=args.degrid_kernel, grid_phases=args.grid_phases)
+                  kernel=args.kernel, degrid_kernel=args.degrid_kernel, grid_phases=args.grid_phases,
+                  grid_cols=args.grid_cols)
     if not args.no_config1:
         c1 = {v: bench_config1(ctx, v, args.config1_reps, tuning) for v in ("reference", "wide")}
         result["config1"] = c1
