@@ -1734,7 +1734,7 @@ __global__ __launch_bounds__(256) void k_degrid_lds(ItemSource<BATCH> src)
 // last column gets one pass with a lane per row instead.  Index arithmetic is
 // 24-bit (full-rate), and only the last step of an entry checks for lanes
 // past its end.
-// NWD waves a workgroup: 64 KB tiles (two workgroups a CU) run eight when the
+// NWD waves a workgroup: 8192-cell tiles (66.5 KB, two workgroups a CU) run eight when the
 // job's C-slice phases stay near the L2's size (see degrid_tiled_t)
 template <int TW, int TH, bool HALF, bool BATCH, int NWD = 4>
 __global__ __launch_bounds__(NWD * 64) void k_degrid_lds2(ItemSource<BATCH> src)
@@ -2423,7 +2423,7 @@ tcv_status grid_tiled_t(tcv_ctx *ctx, const ItemSource<BATCH> &src, int kern_kin
     return TCV_OK;
 }
 
-// manyWaves: the pair degridder on 64 KB tiles runs eight-wave workgroups.
+// manyWaves: the pair degridder on 8192-cell tiles runs eight-wave workgroups.
 // Measured on 128 x 64 tiles: 16 % faster with 3.8 MB phases (spectral
 // workload) and on the uneven batch, 22 % slower with 70 MB phases (a 160 000
 // visibility job cut into 4 phases), where more waves only add misses.
@@ -2532,7 +2532,7 @@ tcv_status launch_grid_tiled(tcv_ctx *ctx, tcv_plan *p, const void *data, const 
     return tcv_fail(ctx, TCV_ERR_UNSUPPORTED, "tile shape not built");
 }
 
-// C-slice phase size up to which the 64 KB-tile degridder runs eight waves
+// C-slice phase size up to which the 8192-cell-tile degridder runs eight waves
 constexpr double kManyWavesPhaseBytes = 32.0 * (1 << 20);
 
 tcv_status launch_degrid_tiled(tcv_ctx *ctx, tcv_plan *p, const float2 *grid, const void *C,
