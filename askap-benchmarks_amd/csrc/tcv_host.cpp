@@ -1141,18 +1141,29 @@ tcv_status tcv_autotune(tcv_ctx *ctx, const tcv_geometry *g, const float *d_data
                 if ((st = consider(t)) != TCV_OK) return st;
             }
         const tcv_tuning b1 = *best;
-        // phase counts around the defaults (multiples of 8 from 8 up)
+        // phase counts from 0.6x to 4x the defaults (multiples of 8 from 8 up), the
+        // gridder's first, then the degridder's: the defaults follow the table's
+        // size, and a job that uses a part of its table (w-planes of a narrower
+        // range than the table spans) keeps its slices in the L2s with finer phases
         auto near = [](int v, double f) {
             const int x = std::max(1, (int)(v * f + 0.5));
             return x >= 8 ? x / 8 * 8 : x;
         };
-        for (int p : {near(b1.phases, 0.75), b1.phases, near(b1.phases, 1.25)})
-            for (int gp : {near(p, 0.75), near(p, 0.6), p}) {
-                if (p > b1.slice_keys || gp > b1.slice_keys || (p == b1.phases && gp == b1.grid_phases))
-                    continue;
+        if (op & 1)
+            for (double f : {0.6, 0.75, 1.5, 2.0, 3.0, 4.0}) {
+                const int gp = near(b1.grid_phases, f);
+                if (gp > b1.slice_keys || gp == b1.grid_phases) continue;
                 tcv_tuning t = b1;
-                t.phases = p;
                 t.grid_phases = gp;
+                if ((st = consider(t)) != TCV_OK) return st;
+            }
+        const tcv_tuning bg = *best;
+        if (op & 2)
+            for (double f : {0.75, 1.25, 2.0, 4.0}) {
+                const int p = near(bg.phases, f);
+                if (p > bg.slice_keys || p == bg.phases) continue;
+                tcv_tuning t = bg;
+                t.phases = p;
                 t.xcd_split = (p >= 8 && p % 8 == 0) ? 1 : 0;
                 if ((st = consider(t)) != TCV_OK) return st;
             }

@@ -286,8 +286,9 @@ tcv_status tcv_plan_info(tcv_ctx *ctx, tcv_plan *plan, tcv_tuning *tuning,
  * the fastest in *best (and its average milliseconds in *bestMs).  op: 1 =
  * gridding, 2 = degridding, 3 = both (summed).  candidates == NULL runs a
  * staged search from the defaults: tile shapes suited to the kernel width x
- * the kernel families, then the degridder's and gridder's C-slice phase
- * counts around the L2-sized target, then the work-item size.  d_grid and
+ * the kernel families, the pair gridder's column groups, then the gridder's
+ * and the degridder's C-slice phase counts from 0.6x to 4x the defaults (a job
+ * that uses part of its table wants finer phases), then the work-item size.  d_grid and
  * d_out are scratch (their contents are overwritten).  The winner is cached in the context for the
  * workload class (grid size, widest kernel, table size, visibility count in
  * powers of two) and used by tcv_plan_create whenever tuning is NULL. Blocking. */
