@@ -167,6 +167,7 @@ struct tcv_plan {
     int *itemCount = nullptr;     // phases * nTiles + 1
     int *itemStart = nullptr;     // phases * nTiles + 1
     int *meta = nullptr;          // 32 ints: [0] items [1] queue head [2] entries [3] bad,
+                                  // [4..5] the entries' cOffset range (see note_c_range),
                                   // [8..15] per-XCD heads, [16..24] per-XCD item ranges
     // the gridder's work items (phasesG); the same buffers as above when
     // phasesG == phases

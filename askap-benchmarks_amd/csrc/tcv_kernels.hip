@@ -473,6 +473,21 @@ struct PlanDevArgs {
     long long cSize;
 };
 
+// The cOffset range of the binned entries, for phases over the part of the
+// table in use: meta[4] = max(0x7fffffff - c0), meta[5] = max(c0 + 1), so the
+// zeroed counters read as an empty range.  One atomic pair per wave.
+__device__ __forceinline__ void note_c_range(int cLo, int cHi, int *meta)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        cLo = min(cLo, __shfl_xor(cLo, o, 64));
+        cHi = max(cHi, __shfl_xor(cHi, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0 && cHi >= 0) {
+        atomicMax(reinterpret_cast<unsigned *>(meta + 4), 0x7fffffffu - (unsigned)cLo);
+        atomicMax(reinterpret_cast<unsigned *>(meta + 5), (unsigned)cHi + 1u);
+    }
+}
+
 // pass 0 counts (tile, slice-key) buckets; pass 1 scatters the entries.
 // A visibility with footprint [u0, u0+s) x [v0, v0+s) is entered once in every
 // tile its footprint overlaps.  Entry = {u0 | v0 << 16, cOffset, vis, s}.
@@ -484,11 +499,13 @@ __global__ __launch_bounds__(256) void k_plan_bin(int pass, GeomDev g, PlanDevAr
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     int isBad = 0;
+    int cLo = 0x7fffffff, cHi = -1;
     if (d < g.nVis) {
         const VisGeom vg = vis_geom(g, d);
         if (!vg.ok) {
             isBad = 1;
         } else {
+            cLo = cHi = vg.c0;
             const int key2 = (int)(((long long)vg.c0 * a.k2) / a.cSize);
             const int tx0 = vg.u0 / a.tw, tx1 = (vg.u0 + vg.s - 1) / a.tw;
             const int ty0 = vg.v0 / a.th, ty1 = (vg.v0 + vg.s - 1) / a.th;
@@ -509,6 +526,7 @@ __global__ __launch_bounds__(256) void k_plan_bin(int pass, GeomDev g, PlanDevAr
     if (pass == 0) {
         const int nb = __builtin_popcountll(__ballot(isBad));
         if ((threadIdx.x & 63) == 0 && nb) atomicAdd(&meta[3], nb);
+        note_c_range(cLo, cHi, meta);
     }
 }
 
@@ -535,6 +553,7 @@ __global__ __launch_bounds__(256) void k_plan_bin_lds(int pass, GeomDev g, PlanD
     __syncthreads();
     const int base = blockIdx.x * blockDim.x * kBinVPT;
     int isBad = 0;
+    int cLo = 0x7fffffff, cHi = -1;
     for (int r = 0; r < kBinVPT; ++r) {
         const int d = base + r * blockDim.x + threadIdx.x;
         if (d >= g.nVis) break;
@@ -543,6 +562,8 @@ __global__ __launch_bounds__(256) void k_plan_bin_lds(int pass, GeomDev g, PlanD
             isBad++;
             continue;
         }
+        cLo = min(cLo, vg.c0);
+        cHi = max(cHi, vg.c0);
         const int key2 = (int)(((long long)vg.c0 * a.k2) / a.cSize);
         const int tx0 = vg.u0 / a.tw, tx1 = (vg.u0 + vg.s - 1) / a.tw;
         const int ty0 = vg.v0 / a.th, ty1 = (vg.v0 + vg.s - 1) / a.th;
@@ -557,6 +578,7 @@ __global__ __launch_bounds__(256) void k_plan_bin_lds(int pass, GeomDev g, PlanD
         int nb = isBad;
         for (int o = 32; o > 0; o >>= 1) nb += __shfl_xor(nb, o, 64);
         if ((threadIdx.x & 63) == 0 && nb) atomicAdd(&meta[3], nb);
+        note_c_range(cLo, cHi, meta);
         return;
     }
     for (int k = threadIdx.x; k < nKeys; k += blockDim.x)
@@ -626,11 +648,18 @@ __global__ __launch_bounds__(256) void k_plan_tile_keys(PlanDevArgs a, const int
 // Work items: each (phase, tile) range of entries is cut into chunks of at most
 // `chunk` entries.  Items are ordered phase-major so concurrently running
 // workgroups read neighbouring C slices.  Item = {tile, first, count, exclusive}.
+// The phases split the keys the entries use (from the cOffset range the
+// binning noted in meta[4..5]) into equal runs, so a job that uses a part of
+// its table (w-planes of a narrower range than the table spans) gets phases of
+// that part's size.  Keys are floor(cOffset * k2 / cSize) by an integer
+// division (one-level binning) or an FP64 product (two-level): both are
+// monotone in cOffset, and the range covers what either gives at its ends.
 __global__ __launch_bounds__(256) void k_plan_items(int pass, PlanDevArgs a,
                                                     const int *__restrict__ starts,
                                                     int *__restrict__ itemCount,
                                                     const int *__restrict__ itemStart,
-                                                    int4 *__restrict__ items)
+                                                    int4 *__restrict__ items,
+                                                    const int *__restrict__ meta)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= a.phases * a.nTiles) return;
@@ -638,8 +667,18 @@ __global__ __launch_bounds__(256) void k_plan_items(int pass, PlanDevArgs a,
     // XCD-major slot order: sub-range p goes to XCD p % 8, so each XCD's items
     // are contiguous and its L2 only sees its share of a phase's C slices
     const int id = a.xcd ? ((p & 7) * (a.phases >> 3) + (p >> 3)) * a.nTiles + t : gid;
-    const int k0 = (int)((long long)p * a.k2 / a.phases);
-    const int k1 = (int)((long long)(p + 1) * a.k2 / a.phases);
+    const int cLo = (int)(0x7fffffffu - (unsigned)meta[4]), cHi = meta[5] - 1;
+    int kLo = 0, kHi = a.k2 - 1;
+    if (cLo <= cHi) {
+        const double scale = (double)a.k2 / (double)a.cSize;
+        auto keyDiv = [&](int c) { return (int)(((long long)c * a.k2) / a.cSize); };
+        auto keyMul = [&](int c) { return min((int)((double)c * scale), a.k2 - 1); };
+        kLo = max(0, min(keyDiv(cLo), keyMul(cLo)));
+        kHi = min(a.k2 - 1, max(keyDiv(cHi), keyMul(cHi)));
+    }
+    const long long span = kHi + 1 - kLo;
+    const int k0 = kLo + (int)((long long)p * span / a.phases);
+    const int k1 = kLo + (int)((long long)(p + 1) * span / a.phases);
     const long long b0 = (long long)t * a.k2;
     const int s = starts[b0 + k0], e = starts[b0 + k1];
     const int n = e - s;
@@ -656,7 +695,7 @@ __global__ __launch_bounds__(256) void k_plan_items(int pass, PlanDevArgs a,
     }
 }
 
-// meta: [0] items [1] queue head [2] entries [3] out of range
+// meta: [0] items [1] queue head [2] entries [3] out of range [4..5] cOffset range
 //       [8..15] per-XCD queue heads, [16..24] per-XCD item ranges (XCD-major plans)
 __global__ void k_plan_meta(const int *__restrict__ itemStart, int nItemSlots,
                             const int *__restrict__ starts, long long nKeys, int xcdSlots,
@@ -2299,12 +2338,14 @@ tcv_status plan_build_kernels(tcv_ctx *ctx, tcv_plan *p, const GeomDev &g)
         b.xcd = xcd;
         const int slots = phases * sh.nTiles;
         hipLaunchKernelGGL(k_plan_items, dim3((slots + 255) / 256), dim3(256), 0, ctx->stream, 0,
-                           b, p->starts, itemCount, (const int *)itemStart, itemsOut);
+                           b, p->starts, itemCount, (const int *)itemStart, itemsOut,
+                           (const int *)p->meta);
         TCV_HIP(ctx, hipGetLastError());
         tcv_status s2 = scan_exclusive(ctx, itemCount, itemStart, slots, p->scanTmp);
         if (s2 != TCV_OK) return s2;
         hipLaunchKernelGGL(k_plan_items, dim3((slots + 255) / 256), dim3(256), 0, ctx->stream, 1,
-                           b, p->starts, itemCount, (const int *)itemStart, itemsOut);
+                           b, p->starts, itemCount, (const int *)itemStart, itemsOut,
+                           (const int *)p->meta);
         TCV_HIP(ctx, hipGetLastError());
         hipLaunchKernelGGL(k_plan_meta, dim3(1), dim3(64), 0, ctx->stream, (const int *)itemStart,
                            slots, (const int *)p->starts, sh.nKeys,

@@ -234,7 +234,8 @@ typedef struct {
                         only); 0 = auto */
     int chunk;       /* max entries per work item (0 = auto) */
     int slice_keys;  /* order entries in a tile by C-slice key, 0 = no ordering */
-    int phases;      /* >1: split each tile's entries into C-slice phases (0 = auto) */
+    int phases;      /* >1: split each tile's entries into C-slice phases: equal runs of
+                        the slice keys the job's entries use (0 = auto) */
     int xcd_split;   /* 1: order work items XCD-major (phases % 8 == 0), one queue per
                         XCD so each XCD's L2 holds its share of a phase's C slices;
                         0 = one queue; -1 = auto */
