@@ -664,6 +664,33 @@ def test_pair_kernels_first_and_last_slice_exact(ctx, Sz, keys):
                                   gref), (tw, th, cols)
 
 
+@pytest.mark.parametrize("band", [(0.40, 0.45), (0.0, 0.02), (0.97, 1.0), (0.5, 0.5)])
+def test_phases_over_the_keys_in_use_exact(ctx, band):
+    """Phases split the slice keys the entries use: entries whose C slices lie
+    in one band of the table (the middle, the start, the end, one slice) are
+    cut into 1 to 64 phases, more phases than used keys included, over the
+    three binnings (two-level with 16 keys a tile and with 4096, more keys than
+    slices; one-level (tile, key) with global atomics on 8192 tiles); every
+    entry lands in exactly one work item: bit-exact."""
+    Sz = 9
+    for keys, tw, th, gSize in ((16, 64, 64, 256), (4096, 128, 64, 256), (8, 16, 32, 2048)):
+        iu, iv, co, data, C, G = exact_problem(1200 + int(100 * band[0]) + keys, 2000, gSize, Sz,
+                                               cluster=None)
+        n_slices = C.size // (Sz * Sz)
+        lo = min(int(band[0] * n_slices), n_slices - 1)
+        hi = max(lo, min(int(band[1] * n_slices), n_slices - 1))
+        rng = np.random.default_rng(int(1000 * band[1]) + keys)
+        co = (rng.integers(lo, hi + 1, iu.size) * Sz * Sz).astype(np.int32)
+        vis = vis_from_host(ctx, iu, iv, co, [Sz], gSize, C.size)
+        gref = O.grid(data, iu, iv, co, C, gSize, Sz)
+        oref = O.degrid(G, iu, iv, co, C, Sz)
+        for phases in (1, 3, 8, 64):
+            tuning = dict(tile_w=tw, tile_h=th, slice_keys=keys, phases=min(phases, keys),
+                          grid_phases=min(phases, keys), chunk=64)
+            assert np.array_equal(gpu_grid(ctx, vis, data, C, gSize, "tiled", tuning), gref), tuning
+            assert np.array_equal(gpu_degrid(ctx, vis, G, C, "tiled", tuning), oref), tuning
+
+
 @pytest.mark.parametrize("tw,th,keys", [(16, 32, 4), (64, 64, 1), (64, 64, 16)])
 def test_plan_binning_paths_exact(ctx, tw, th, keys):
     """The three plan binnings give the same (bit-exact) results: (tile, key)
