@@ -39,6 +39,10 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, published
 # (attic/tConvolveCuda/slurm-2759.out:16-37), M grid points/s
 P100_GRID_MPTS = 14013.5
 P100_DEGRID_MPTS = 38036.6
+# published per-process rates of the current generation's tConvolveMPI on one
+# Magnus node with one process (README.md:148-184), (gridding, degridding) Mpix/s,
+# for runType 0 (continuum) and 1 (spectral)
+MAGNUS_NP1_MPIX = {0: (294.649, 206.756), 1: (482.345, 390.47)}
 # the GPU box grants one GPU's job this many host CPUs, whatever the machine has
 BOX_CPU_SHARE = 16
 
@@ -423,6 +427,77 @@ def bench_spectral(ctx, args, tuning, steps):
     return out
 
 
+# ---- the current generation's runType 0 and 1 --------------------------------------
+
+def bench_current(ctx, reps):
+    """current/tConvolveMPI's two reported runs on one GPU: runType 0 (continuum,
+    6440^2 grid, 483 w-planes of 7-121 wide kernels) and 1 (spectral), the
+    reference's own uvw generator, C(w) and setSort(0), each gridding and
+    degridding timed apart as tConvolveMPI.cc:101-170 times them, in Mpix/s (its
+    nPixelsGridded over the time) beside the published per-process rates and the
+    requiredRate model (Benchmark.cc:702-740).  Kernel time with the plan held;
+    the plan's re-binning of new u, v, w is timed beside it."""
+    import tcv
+    req = tcv.CurrentBenchmark.requiredRate()
+    out = {}
+    for rt in (0, 1):
+        bm = tcv.CurrentBenchmark(ctx)
+        bm.setRunType(rt)
+        bm.init()
+        npp = bm.numPerPlane.astype(np.int64)
+        s = bm.sSize.astype(np.int64)
+        pix = int((npp * s * s).sum())
+        bm.runGrid()
+        bm.runDegrid()
+        ctx.sync()
+
+        def timed(fn):
+            ctx.sync()
+            ctx.timer_start()
+            for _ in range(reps):
+                fn()
+            return ctx.timer_stop() / reps
+
+        t_plan = timed(lambda: bm._plan().build())
+        t_grid = timed(bm.runGrid)
+        t_degrid = timed(bm.runDegrid)
+        # the library's autotuner on the same job, in a context of its own
+        actx = tcv.Context(ctx.device)
+        try:
+            sg = ctx.zeros((bm.gSize, bm.gSize), np.complex64)
+            so = ctx.empty(bm.samples.nVis, np.complex64)
+            best = tcv.autotune(actx, bm.samples, bm.data, bm.C, sg, so, op=3)
+            sg.free()
+            so.free()
+            tuned = {k: best[k] for k in TUNING_KEYS + ("tap_bytes", "item_entries")}
+        finally:
+            actx.close()
+        plan_a = tcv.Plan(ctx, bm.samples, **tuned)
+        t_grid_a = timed(lambda: plan_a.grid(bm.data, bm.C, bm.grid1))
+        t_degrid_a = timed(lambda: plan_a.degrid(bm.grid1, bm.C, bm.outdata1))
+        plan_a.close()
+        mpix = lambda t: pix / (t * 1e-3) / 1e6
+        pub = MAGNUS_NP1_MPIX[rt]
+        out[f"runType{rt}"] = {
+            "visibilities": int(bm.nVisibilitiesGridded()), "pixels_gridded": pix,
+            "out_of_range": int(bm.nOutOfRange), "gSize": int(bm.gSize), "wSize": int(bm.wSize),
+            "plan_ms": t_plan, "grid_ms": t_grid, "degrid_ms": t_degrid,
+            "grid_Mpix_per_s": mpix(t_grid), "degrid_Mpix_per_s": mpix(t_degrid),
+            "vs_magnus_process_grid": mpix(t_grid) / pub[0],
+            "vs_magnus_process_degrid": mpix(t_degrid) / pub[1],
+            "autotuned": {"grid_ms": t_grid_a, "degrid_ms": t_degrid_a,
+                          "grid_Mpix_per_s": mpix(t_grid_a), "degrid_Mpix_per_s": mpix(t_degrid_a),
+                          "tuning": tuned},
+            "required_Mpix_per_s": req[rt] / 1e6,
+            "tuning": {k: bm._plan().info()[k] for k in TUNING_KEYS}}
+        log(f"current runType {rt}: grid {mpix(t_grid):.0f} Mpix/s ({mpix(t_grid) / pub[0]:.0f}x a "
+            f"Magnus process), degrid {mpix(t_degrid):.0f} ({mpix(t_degrid) / pub[1]:.0f}x); autotuned "
+            f"{mpix(t_grid_a):.0f} / {mpix(t_degrid_a):.0f}; plan {t_plan:.2f} ms; required "
+            f"{req[rt] / 1e6:.1f} Mpix/s")
+        bm.plan.close()
+    return out
+
+
 # ---- major cycle ----------------------------------------------------------------
 
 def bench_major(ctx, args):
@@ -701,6 +776,7 @@ def main():
     ap.add_argument("--serve-exclusive-points", type=float, default=1e10,
                     help="requests of at least this many grid points run alone (0: never)")
     ap.add_argument("--no-major", action="store_true")
+    ap.add_argument("--no-current", action="store_true")
     ap.add_argument("--major-samples", type=int, default=160000)
     ap.add_argument("--major-cycles", type=int, default=5)
     ap.add_argument("--major-minor", type=int, default=100)
@@ -766,6 +842,10 @@ def main():
         result["p100_degrid_Mpts"] = P100_DEGRID_MPTS
     if not args.no_spectral:
         result["spectral"] = bench_spectral(ctx, args, tuning, args.spectral_steps)
+    # the current generation's runs sit with config 1 (the other per-process
+    # comparison): --no-config1 leaves both out
+    if not args.no_config1 and not args.no_current:
+        result["current"] = bench_current(ctx, args.config1_reps)
     if not args.no_major:
         result["major_cycle"] = bench_major(ctx, args)
     print(json.dumps(result), flush=True)
