@@ -1862,17 +1862,18 @@ __global__ __launch_bounds__(NWD * 64) void k_degrid_lds2(ItemSource<BATCH> src)
             // (base + 2 p TB) + rl * (sB - 2 wp TB): one 24-bit multiply-add each.
             // aL >= 0; aC < 0 when a kernel narrower than the tile gets its added column
             const int aC = sB - 2 * (int)TB * wp, aL = RS * 8 - 8 * wp;
-            auto step = [&](int p0, auto tailTag, auto modeTag, auto oddTag) {
+            auto step = [&](int p0, auto tailTag, auto modeTag, auto oddTag, auto uTag) {
                 constexpr bool TAIL = decltype(tailTag)::value;
                 constexpr int MODE = decltype(modeTag)::value;
                 constexpr bool ODD = decltype(oddTag)::value;   // pairs start on odd stored columns
-                float4 c[kLdsU];
-                float2 g0[kLdsU], g1[kLdsU];
+                constexpr int U = decltype(uTag)::value;        // pairs a lane takes
+                float4 c[U];
+                float2 g0[U], g1[U];
                 const int pl = p0 + lane;
                 const float pf = (float)pl + 0.5f;
                 const unsigned cP = cB + (unsigned)pl * (2u * TB), lP = lB + (unsigned)pl * 8u;
 #pragma unroll
-                for (int u = 0; u < kLdsU; ++u) {
+                for (int u = 0; u < U; ++u) {
                     // row = floor((p + 0.5) / wp): p + 0.5 is at least 1/(2 wp) from a
                     // multiple of wp, far beyond the rounding of the float arithmetic
                     const int rl = (int)fmaf(pf, rw, (float)(64 * u) * rw);
@@ -1904,7 +1905,7 @@ __global__ __launch_bounds__(NWD * 64) void k_degrid_lds2(ItemSource<BATCH> src)
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < kLdsU; ++u) {
+                for (int u = 0; u < U; ++u) {
                     sr = fmaf(g0[u].x, c[u].x, fmaf(-g0[u].y, c[u].y, sr));
                     si = fmaf(g0[u].x, c[u].y, fmaf(g0[u].y, c[u].x, si));
                     sr = fmaf(g1[u].x, c[u].z, fmaf(-g1[u].y, c[u].w, sr));
@@ -1912,9 +1913,20 @@ __global__ __launch_bounds__(NWD * 64) void k_degrid_lds2(ItemSource<BATCH> src)
                 }
             };
             auto walk = [&](auto modeTag, auto oddTag) {
+                using Full = std::integral_constant<int, kLdsU>;
                 int p0 = 0;
-                for (; p0 + 64 * kLdsU <= n; p0 += 64 * kLdsU) step(p0, std::false_type{}, modeTag, oddTag);
-                if (p0 < n) step(p0, std::true_type{}, modeTag, oddTag);
+                for (; p0 + 64 * kLdsU <= n; p0 += 64 * kLdsU) step(p0, std::false_type{}, modeTag, oddTag, Full{});
+                // the last pairs with only as many loads unrolled as they fill:
+                // a narrow kernel's whole footprint is one such step, and the
+                // idle lanes of unneeded loads cost their address arithmetic
+                const int r = n - p0;
+                if (r <= 0) return;
+                if (r <= 64) step(p0, std::true_type{}, modeTag, oddTag, std::integral_constant<int, 1>{});
+                else if (r <= 128 && kLdsU > 2)
+                    step(p0, std::true_type{}, modeTag, oddTag, std::integral_constant<int, (kLdsU > 2 ? 2 : 1)>{});
+                else if (r <= 256 && kLdsU > 4)
+                    step(p0, std::true_type{}, modeTag, oddTag, std::integral_constant<int, (kLdsU > 4 ? 4 : 1)>{});
+                else step(p0, std::true_type{}, modeTag, oddTag, Full{});
             };
             auto walkMode = [&](auto oddTag) {
                 if (mode == 1) walk(std::integral_constant<int, 1>{}, oddTag);
