@@ -34,6 +34,7 @@ sys.path.insert(0, os.path.join(REPO, "askap-benchmarks_amd"))
 sys.path.insert(0, os.path.join(REPO, "oracle"))
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, published
+ENGINE_CLOCK_GHZ = 2.4  # MI355X peak engine clock, published
 # published GPU figures for this path: Tesla P100, CUDA d_gridKernel and
 # d_degridKernel<64>, config 1 with u, v, w in [-1000, 1000] m
 # (attic/tConvolveCuda/slurm-2759.out:16-37), M grid points/s
@@ -101,7 +102,7 @@ def calibration_copy(ctx, nbytes=1 << 30, reps=2):
     b.free()
 
 
-def roofline(kernel, avg_ms, bytes_alg, ceilings, rec):
+def roofline(kernel, avg_ms, bytes_alg, ceilings, rec, cus):
     """Roofline of one kernel.  The C taps of these workloads are reused from
     the L2 and Infinity Cache (each slice serves hundreds to thousands of
     visibilities), so the bytes that must cross HBM are far fewer than the 8 B
@@ -109,7 +110,10 @@ def roofline(kernel, avg_ms, bytes_alg, ceilings, rec):
     CUs' vector-memory pipeline).  `achieved` = algorithmic bytes / launch
     time against the measured L2 read ceiling (MI355X publishes none); the
     HBM traffic the counters saw (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
-    and the L2 -> L1 bytes (TCP_TCC_READ_REQ x 128 B) stand beside it."""
+    and the L2 -> L1 bytes (TCP_TCC_READ_REQ x 128 B) stand beside it, and so
+    does the share of the launch the CUs' texture address units were busy
+    (TA_TA_BUSY summed over the CUs, at the peak engine clock): the unit that
+    binds these kernels."""
     achieved = bytes_alg / (avg_ms * 1e-3) / 1e9
     traffic = (rec.get("bytes_per_launch") or {}).get(kernel)
     l2 = (rec.get("l2_read_bytes_per_launch") or {}).get(kernel)
@@ -128,6 +132,10 @@ def roofline(kernel, avg_ms, bytes_alg, ceilings, rec):
         out["l2_to_l1"] = {"bytes_per_launch": l2, "GBps": l2 / (avg_ms * 1e-3) / 1e9,
                            "frac_of_l2_ceiling": l2 / (avg_ms * 1e-3) / 1e9 / ceilings["l2"],
                            "over_algorithmic": l2 / bytes_alg}
+    ta = (rec.get("ta_busy_cycles_per_launch") or {}).get(kernel)
+    if ta:
+        out["address_unit"] = {"busy_cycles_sum": ta, "cus": cus, "clock_GHz": ENGINE_CLOCK_GHZ,
+                               "busy_frac": ta / (cus * avg_ms * 1e-3 * ENGINE_CLOCK_GHZ * 1e9)}
     return out
 
 
@@ -185,7 +193,7 @@ def bench_batch(ctx, args, ceilings):
         rec = {}
     rf = roofline(dom, kern[dom]["avg_ms"],
                   alg_bytes(bj.points, nVis, cells, "grid" if dom == "grid_batch" else "degrid"),
-                  ceilings, rec)
+                  ceilings, rec, ctx.cus)
 
     def timed(fn, reps):
         ctx.sync()
@@ -385,7 +393,7 @@ def bench_spectral(ctx, args, tuning, steps):
     if rec and all(rec.get("tuning", {}).get(k) == info.get(k) for k in TUNING_KEYS):
         out["roofline"] = {k: roofline(k, kern[k]["avg_ms"],
                                        alg_bytes(points, prob.nVis, prob.gSize ** 2, k.split("_")[0]),
-                                       args._ceilings, rec)
+                                       args._ceilings, rec, ctx.cus)
                            for k in ("grid_tiled", "degrid_tiled")}
     log(f"spectral: {ms_step:.2f} ms/step, {value / 1e9:.1f} G points/s (grid "
         f"{kern['grid_tiled']['avg_ms']:.1f}, degrid {kern['degrid_tiled']['avg_ms']:.1f} ms)")
