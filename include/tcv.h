@@ -232,7 +232,8 @@ typedef struct {
     int tile_h;      /* rows per tile: 32 or 64 for tiles 8..64 wide, 64 for 128-wide
                         tiles, and the tall shapes 64 x 128 and 32 x 256 (pair kernels
                         only); 0 = auto */
-    int chunk;       /* max entries per work item (0 = auto) */
+    int chunk;       /* max entries per work item (0 = auto: 2048, 1024 for jobs with
+                        kernels narrower than 24, 512 when all are) */
     int slice_keys;  /* order entries in a tile by C-slice key, 0 = no ordering */
     int phases;      /* >1: split each tile's entries into C-slice phases: equal runs of
                         the slice keys the job's entries use (0 = auto) */
