@@ -518,11 +518,12 @@ static void auto_tuning(int maxS, const tcv_geometry *g, tcv_tuning *t)
     if (t->tile_h <= 0) t->tile_h = 64;
     if (t->grid_cols <= 0) t->grid_cols = auto_grid_cols(t->tile_w, t->tile_h);
     if (t->chunk <= 0)
-        // narrow kernels make short entries, and work items of 2048 of them ran long
-        // enough to leave workgroups idle at a launch's end: 512 for jobs of narrow
-        // kernels only (current runType 1, widths 7-19: grid 31 % and degrid 18 %
-        // faster than 2048), 1024 for jobs with some (runType 0, widths 7-121: 11 % and
-        // 5 %); wide kernels keep 2048 (config 1's degridder is 6 % slower at 512)
+        // narrow kernels make short entries, and shorter work items measured
+        // faster for them (more items to share out, none running long at the
+        // end): 512 for jobs of narrow kernels only (current runType 1, widths
+        // 7-19: grid 31 % and degrid 18 % faster than 2048), 1024 for jobs with
+        // some (runType 0, widths 7-121: 10 % and 3 %); wide kernels keep 2048
+        // (config 1's degridder is 6 % slower at 512)
         t->chunk = maxS < 24 ? 512 : minS < 24 ? 1024 : 2048;
     const long long slices = std::max<long long>(1, g->cSize / ((long long)minS * minS));
     if (t->slice_keys <= 0) t->slice_keys = (int)std::min<long long>(2048, slices);
